@@ -1,0 +1,154 @@
+/* prc_d2z.h -- C ABI of the MI355X delta2zreion library (libprc_d2z.so).
+ *
+ * The library replaces two seams of pr-calc's d2z program
+ * (marcelo-alvarez/pr-calc):
+ *   1. the FFTW3-MPI set-up and forward transform in c++/src/main_d2z.C:32-48
+ *      (fftwf_mpi_local_size_3d, AllocateArrays, plan creation) and :52-67
+ *      (delta /= DInit, fftwf_execute(plan));
+ *   2. void delta2zreion() (c++/src/proto_d2z.h:27, c++/src/delta2zreion_d2z.C:8-216):
+ *      sigma normalisation, erfc^-1(1/zeta), sigma_min, and for every radius the
+ *      top-hat filter, c2r, 1/N^3, sigma_grid, sigma(R), barrier and running max.
+ * The host keeps the parameter file, command line, ICs input, history and
+ * zreion output.  One context owns the device buffers of one box; the whole
+ * grid lives on one GPU (no MPI, no slab decomposition).
+ *
+ * Conventions: every call returns PRC_D2Z_OK (0) or a negative status and never
+ * prints or exits; prc_d2z_last_error() gives the message.  Host buffers are
+ * caller-owned.  Calls on one context are synchronous unless named _async and
+ * must not overlap; different contexts are independent.
+ */
+#ifndef PRC_D2Z_H
+#define PRC_D2Z_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRC_D2Z_ABI_VERSION 1
+#define PRC_D2Z_NHISTORY 100000 /* NHISTORY, c++/src/allvars_d2z.h:1 */
+
+typedef struct prc_d2z_ctx prc_d2z_ctx;
+
+enum prc_d2z_status {
+  PRC_D2Z_OK = 0,
+  PRC_D2Z_ERR_BAD_ARG = -1,
+  PRC_D2Z_ERR_UNSUPPORTED_N = -2,
+  PRC_D2Z_ERR_OOM = -3,
+  PRC_D2Z_ERR_HIP = -4,
+  PRC_D2Z_ERR_NAN_INPUT = -5,
+  PRC_D2Z_ERR_STATE = -6,
+  PRC_D2Z_ERR_NO_DEVICE = -7
+};
+
+/* Execution paths over the same arithmetic contract. */
+enum prc_d2z_mode {
+  PRC_D2Z_PLAIN_FP32 = 0, /* separate kernels: filter, 3 c2r passes, barrier, moments */
+  PRC_D2Z_FUSED_FP32 = 1, /* filter fused into the first inverse pass, barrier/max into the last,
+                             radii_per_launch radii per launch off one resident spectrum */
+  PRC_D2Z_FUSED_MIXED = 2 /* fused, with 16-bit storage of the inter-pass intermediates */
+};
+
+/* Real-space field layouts of the reference: dense N^3 floats (ICs and .zreion
+ * files, c++/src/io_d2z.C:164-246) or the in-place FFTW layout float[N][N][N+2]
+ * (c++/src/allocate_d2z.C:27-37). */
+enum prc_d2z_layout { PRC_D2Z_DENSE = 0, PRC_D2Z_PADDED = 1 };
+
+typedef struct {
+  /* parameter-file tags, c++/src/io_d2z.C:33-87 (struct Parameter,
+   * c++/src/allvars_d2z.h:58-64); BoxSize in Mpc/h as in the file */
+  float Omegam, Omegab, Omegal, h, ns, sigma8, w;
+  float BoxSize, zInit, Rmin;
+  int N, Nscales;
+  int RecordBubbleSizes;
+  /* command-line values (struct clParameter, c++/src/allvars_d2z.h:66-70;
+   * defaults c++/src/commandline_d2z.C:47-52): Rmax in Mpc/h, M_min in Msun */
+  float Rmax, zeta, M_min;
+  /* execution options (no counterpart in the reference) */
+  int mode;             /* enum prc_d2z_mode */
+  int radii_per_launch; /* fused modes: radii per batch, 1..8; 0 = default */
+  int device;           /* HIP device ordinal */
+} prc_d2z_params;
+
+/* Derived scalars (globals DInit, sigma_min, erfizeta, zHigh, DHigh of
+ * c++/src/allvars_d2z.h:76 plus pnorm and the barrier slope A). */
+typedef struct {
+  float DInit, DHigh, A, pnorm, erfizeta, sigma_min;
+} prc_d2z_scalars;
+
+/* One row of the per-radius table printed by delta2zreion_d2z.C:102-103. */
+typedef struct {
+  float R_mpch;     /* R * h */
+  float sigma;      /* sigma(R) from the EH98 no-wiggle spectrum */
+  float sigma_grid; /* rms of the smoothed grid */
+} prc_d2z_radius_row;
+
+/* Kernel timing of the last prc_d2z_run, from HIP events recorded on the
+ * context's stream around every launch. */
+#define PRC_D2Z_MAX_KERNELS 16
+typedef struct {
+  double total_ms; /* forward transform through the last radius */
+  int nkernels;
+  char name[PRC_D2Z_MAX_KERNELS][32];
+  double ms[PRC_D2Z_MAX_KERNELS]; /* summed over all launches of that kernel */
+  int launches[PRC_D2Z_MAX_KERNELS];
+} prc_d2z_timing;
+
+/* Fills reference defaults (commandline_d2z.C:47-52: Rmax 256, zeta 10,
+ * M_min 1e8) and a 128^3 / 200 Mpc/h / 32-radius box. */
+void prc_d2z_default_params(prc_d2z_params* p);
+
+/* fftwf_mpi_local_size_3d + AllocateArrays + plan creation + the scalar
+ * set-up of delta2zreion (normalize, inverse_erfc, sigma_min, sigma(R)),
+ * computed on the GPU in FP64.  N must be 2^a 3^b with N/2 supported
+ * (16, 32, 64, 128, 192, 256, 512, 1024, 2048). */
+int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out);
+void prc_d2z_destroy(prc_d2z_ctx* ctx);
+/* Message of the last failed call on ctx (ctx == NULL: last failed create on
+ * this thread). */
+const char* prc_d2z_last_error(const prc_d2z_ctx* ctx);
+
+/* ReadDeltaFile's destination (io_d2z.C:164-207): copy the real-space density
+ * contrast (N^3 cells, x slowest) to the device.  Rejects NaN/Inf. */
+int prc_d2z_set_delta(prc_d2z_ctx* ctx, const float* delta, int layout);
+/* Same from device memory already on the context's device. */
+int prc_d2z_set_delta_device(prc_d2z_ctx* ctx, const float* d_delta, int layout);
+
+/* main_d2z.C:57-71: delta /= DInit, forward r2c, then delta2zreion() for all
+ * radii.  zreion starts from 0 (allocate_d2z.C:37) on every call. */
+int prc_d2z_run(prc_d2z_ctx* ctx);
+int prc_d2z_run_async(prc_d2z_ctx* ctx);
+int prc_d2z_sync(prc_d2z_ctx* ctx);
+
+/* Results (WriteZreionFile's source, io_d2z.C:209-246). */
+int prc_d2z_get_zreion(prc_d2z_ctx* ctx, float* out, int layout);
+/* rbubble (RecordBubbleSizes = 1): R in Mpc of the radius that set zreion; 0
+ * where zreion stayed 0. */
+int prc_d2z_get_rbubble(prc_d2z_ctx* ctx, float* out, int layout);
+int prc_d2z_get_scalars(prc_d2z_ctx* ctx, prc_d2z_scalars* out);
+int prc_d2z_get_radius_table(prc_d2z_ctx* ctx, prc_d2z_radius_row* rows, int n);
+/* Device pointer of the dense zreion field (valid until destroy). */
+int prc_d2z_zreion_device(prc_d2z_ctx* ctx, const float** d_zreion);
+
+/* GetHistory (c++/src/bubblehistory_d2z.C:5-41) on the GPU: integer counts per
+ * history bin, bin = (int)((zreion - 49.9)/dzhistory). */
+int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */);
+
+/* Smoothed field delta_R of radius r (filter + c2r + 1/N^3), dense layout,
+ * through the context's transform path. */
+int prc_d2z_smoothed_field(prc_d2z_ctx* ctx, int r, float* out);
+
+/* Per-kernel HIP-event timing of prc_d2z_run (off by default). */
+int prc_d2z_set_timing(prc_d2z_ctx* ctx, int enable);
+int prc_d2z_get_timing(prc_d2z_ctx* ctx, prc_d2z_timing* out);
+
+/* Device buffer bytes this context holds. */
+size_t prc_d2z_device_bytes(const prc_d2z_ctx* ctx);
+int prc_d2z_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRC_D2Z_H */

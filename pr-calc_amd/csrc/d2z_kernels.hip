@@ -1,0 +1,976 @@
+// HIP kernels for the delta2zreion path on gfx950.
+//
+// Data layout in HBM (all row-major, last index fastest):
+//   real fields        float  [N][N][N]           (x, y, z)
+//   half spectrum S    float2 [N][N][N/2]         (kx, ky, kz) with the kz = 0
+//       slot holding the packed pair  H(kz=0) + i*H(kz=N/2)  of the z-transform
+//       (both are real along each z row, so one complex slot carries them);
+//   side planes D0, DN float2 [N][N]              the true spectrum at kz = 0
+//       and kz = N/2, unpacked once after the forward transform.
+//
+// Reference: c++/src/main_d2z.C:57-67 (scale, forward r2c) and
+// c++/src/delta2zreion_d2z.C:8-216 (per-radius filter, c2r, normalise,
+// sigma_grid, barrier, running max).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "fft_core.h"
+#include "d2z_kernels.h"
+
+namespace prc {
+
+// --------------------------------------------------------------- twiddles
+// tw[m] = exp(+2 pi i m / L), m < L, from FP64 sincospi rounded to float.
+__global__ void k_twiddles(float2* __restrict__ tw, int L) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < L) {
+    double s, c;
+    sincospi(2.0 * (double)m / (double)L, &s, &c);
+    tw[m] = make_float2((float)c, (float)s);
+  }
+}
+
+// ---------------------------------------------------- top-hat filter W(kR)
+// W(x) = 3 (sin x - x cos x) / x^3; Taylor series below x = 1 where the closed
+// form cancels (terms to x^10: truncation < 4e-11 on [0,1)).
+__device__ __forceinline__ float tophat(float x) {
+  if (x < 1.0f) {
+    float x2 = x * x;
+    return 1.0f - x2 * (1.0f / 10.0f -
+                        x2 * (1.0f / 280.0f -
+                              x2 * (1.0f / 15120.0f - x2 * (1.0f / 1330560.0f - x2 * (1.0f / 172972800.0f)))));
+  }
+  float s, c;
+  sincosf(x, &s, &c);
+  return 3.0f * (s - x * c) / (x * x * x);
+}
+
+// signed frequency index of grid index i (i >= N/2 maps to i - N), squared
+__device__ __forceinline__ int fsq(int i, int N) {
+  int f = (i >= (N >> 1)) ? i - N : i;
+  return f * f;
+}
+
+// W(|k| R) / N^3 for integer |k|^2 = m dk^2; k = 0 gives 0 (the reference
+// zeroes the mean mode, delta2zreion_d2z.C:79-85).
+__device__ __forceinline__ float filter_coef(int m, float dkR, float invN3) {
+  if (m == 0) return 0.0f;
+  return tophat(sqrtf((float)m) * dkR) * invN3;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Remap so each XCD works on a contiguous range of tiles: neighbouring
+// tiles that share 128 B lines then meet in the same L2.
+__device__ __forceinline__ long long xcd_tile(long long b, long long nblk) {
+  if (nblk % 8 != 0) return b;
+  return (b % 8) * (nblk / 8) + b / 8;
+}
+
+// ------------------------------------------------ forward z pass (r2c)
+// Row of N reals -> packed half spectrum of N/2 complex.  The row is read as
+// N/2 complex z[n] = x[2n] + i x[2n+1] and divided by DInit
+// (main_d2z.C:57-59); after the N/2-point forward FFT Z[k]:
+//   X[k] = (Z[k] + conj Z[L-k])/2 - i exp(-2 pi i k/N) (Z[k] - conj Z[L-k])/2,
+//   slot 0 = X[0] + i X[N/2] = (Re Z0 + Im Z0) + i (Re Z0 - Im Z0).
+template <int L>
+__global__ void __launch_bounds__(256) k_fwd_z_r2c(const float* __restrict__ in, long long in_row_stride,
+                                                   float2* __restrict__ out, long long nrows, float dinit,
+                                                   const float2* __restrict__ tw_half,
+                                                   const float2* __restrict__ tw_full) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  __shared__ float2 smem[ROWS * L];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const long long row = (long long)blockIdx.x * ROWS + rr;
+  const bool valid = row < nrows;
+  LdsPencil lds{smem + rr * L, 1};
+  float2 v[E];
+  const float2* src = reinterpret_cast<const float2*>(in + (valid ? row : 0) * in_row_stride);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    float2 a = valid ? src[i + e * T] : make_float2(0.f, 0.f);
+    v[e] = make_float2(a.x / dinit, a.y / dinit);
+  }
+  pencil_fft<L, -1>(v, i, tw_half, lds);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[i + e * T] = v[e];
+  __syncthreads();
+  float2* dst = out + (valid ? row : 0) * L;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = i + e * T;
+    float2 Z = v[e];
+    float2 X;
+    if (k == 0) {
+      X = make_float2(Z.x + Z.y, Z.x - Z.y);
+    } else {
+      float2 Zm = cconj(lds[L - k]);
+      float2 Ev = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y + Zm.y));
+      float2 D = make_float2(0.5f * (Z.x - Zm.x), 0.5f * (Z.y - Zm.y));
+      float2 Od = make_float2(D.y, -D.x);  // -i * D
+      float2 w = tw_full[k];
+      w.y = -w.y;  // exp(-2 pi i k / N)
+      X = cadd(Ev, cmul(w, Od));
+    }
+    if (valid) dst[k] = X;
+  }
+}
+
+// --------------------------------------------- strided complex pass
+// In-place 1D transform of length L along a non-contiguous axis.  Block =
+// one tile: C consecutive columns (contiguous in memory) x all L elements
+// along the axis (stride sA).  Lane -> column mapping makes every load and
+// store a run of C complex values.
+template <int L, int C, int DIR>
+__global__ void __launch_bounds__(Plan<L>::T* C) k_strided(float2* __restrict__ data, long long sA, long long sO,
+                                                            int nColBlocks, const float2* __restrict__ tw) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+  __shared__ float2 smem[L * C];
+  const int t = threadIdx.x;
+  const int c = t % C, i = t / C;
+  const long long blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int cb = (int)(blk % nColBlocks);
+  const long long o = blk / nColBlocks;
+  float2* base = data + o * sO + (long long)cb * C + c;
+  LdsPencil lds{smem + c, C};
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = base[(long long)(i + e * T) * sA];
+  pencil_fft<L, DIR>(v, i, tw, lds);
+#pragma unroll
+  for (int e = 0; e < E; ++e) base[(long long)(i + e * T) * sA] = v[e];
+}
+
+// ---------------------------------------------------- inverse z pass (c2r)
+// Packed half-spectrum row (N/2 complex) -> N reals, in place.
+//   Z[k] = X[k] + conj X[L-k] + i exp(+2 pi i k/N) (X[k] - conj X[L-k]),
+//   Z[0] = (X0 + XN) + i (X0 - XN),  z = IFFT_L(Z),  x[2n] + i x[2n+1] = z[n].
+template <int L>
+__device__ __forceinline__ void c2r_preprocess(float2 (&v)[Plan<L>::E], const float2* __restrict__ row, int i,
+                                               const float2* __restrict__ tw_full) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = i + e * T;
+    float2 a = v[e];
+    if (k == 0) {
+      v[e] = make_float2(a.x + a.y, a.x - a.y);
+    } else {
+      float2 b = cconj(row[L - k]);
+      float2 s = cadd(a, b), d = csub(a, b);
+      float2 w = tw_full[k];  // exp(+2 pi i k / N)
+      float2 wd = cmul(w, d);
+      v[e] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i * wd
+    }
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) k_inv_z_c2r(float2* __restrict__ data, long long nrows,
+                                                   const float2* __restrict__ tw_half,
+                                                   const float2* __restrict__ tw_full) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  __shared__ float2 smem[ROWS * L];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const long long row = (long long)blockIdx.x * ROWS + rr;
+  const bool valid = row < nrows;
+  float2* rp = data + (valid ? row : 0) * L;
+  LdsPencil lds{smem + rr * L, 1};
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = rp[i + e * T];
+  c2r_preprocess<L>(v, rp, i, tw_full);
+  pencil_fft<L, +1>(v, i, tw_half, lds);
+  __syncthreads();  // every lane of the row has read its inputs before any store
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) rp[i + e * T] = v[e];
+  }
+}
+
+// ------------------------------------------------- unpack after forward
+// D0(k) = (P(k) + conj P(-k))/2,  DN(k) = -i (P(k) - conj P(-k))/2
+__global__ void k_unpack_planes(const float2* __restrict__ S, float2* __restrict__ D0, float2* __restrict__ DN,
+                                int N) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long long)N * N) return;
+  const int kx = (int)(id / N), ky = (int)(id % N);
+  const int mx = (N - kx) % N, my = (N - ky) % N;
+  const long long Nh = N / 2;
+  float2 p = S[((long long)kx * N + ky) * Nh];
+  float2 q = cconj(S[((long long)mx * N + my) * Nh]);
+  D0[id] = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y + q.y));
+  float2 d = make_float2(0.5f * (p.x - q.x), 0.5f * (p.y - q.y));
+  DN[id] = make_float2(d.y, -d.x);
+}
+
+// ------------------------------------------------ plain path: filter
+// T[kx][ky][kz] = W(|k| R)/N^3 * S for kz >= 1; the kz = 0 slot is rebuilt
+// from the side planes as W0 D0 + i WN DN (delta2zreion_d2z.C:58-89 plus the
+// 1/N^3 of NormalizeArray, :93).
+__global__ void k_filter(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+                         float2* __restrict__ out, int N, float dkR, float invN3) {
+  const long long Nh = N / 2;
+  const long long total = (long long)N * N * Nh;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(id % Nh);
+    const long long pl = id / Nh;
+    const int ky = (int)(pl % N), kx = (int)(pl / N);
+    const int mxy = fsq(kx, N) + fsq(ky, N);
+    float2 r;
+    if (kz != 0) {
+      r = cscale(S[id], filter_coef(mxy + kz * kz, dkR, invN3));
+    } else {
+      const float w0 = filter_coef(mxy, dkR, invN3);
+      const float wn = filter_coef(mxy + (int)(Nh * Nh), dkR, invN3);
+      const float2 a = D0[pl], b = DN[pl];
+      r = make_float2(w0 * a.x - wn * b.y, w0 * a.y + wn * b.x);
+    }
+    out[id] = r;
+  }
+}
+
+// -------------------------------------- barrier: d2z() in double, stored float
+// z = A*(delta_R + B) - 1 with A float and B = sqrt(2(smin^2 - s^2))*erfizeta in
+// double (delta2zreion_d2z.C:206-216); no contraction into an FMA, as in the
+// reference's separate multiply and subtract.
+__device__ __forceinline__ float barrier_z(float d, double A, double B) {
+  return (float)__dsub_rn(__dmul_rn(A, __dadd_rn((double)d, B)), 1.0);
+}
+
+// plain path: strict running maximum over a real field (delta2zreion_d2z.C:106-146)
+__global__ void k_barrier(const float* __restrict__ dR, float* __restrict__ zre, float* __restrict__ rbub,
+                          long long total, const RadiusConst* __restrict__ rc, int r) {
+  const double A = rc[r].A;
+  const double B = rc[r].B;
+  const float R = rc[r].R;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const float z = barrier_z(dR[id], A, B);
+    if (z > zre[id]) {
+      zre[id] = z;
+      if (rbub) rbub[id] = R;
+    }
+  }
+}
+
+// ------------------------------------------ sigma_grid partial sums (plain)
+// Per-block FP64 sums of delta and delta^2 (GetMeanSigma, arrayoperations.C:4-35).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_moments(const float* __restrict__ x, long long total,
+                                                 double2* __restrict__ partial) {
+  double s1 = 0.0, s2 = 0.0;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const double d = (double)x[id];
+    s1 += d;
+    s2 += d * d;
+  }
+  __shared__ double sh[2][4];
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) {
+    sh[0][w] = s1;
+    sh[1][w] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) {
+      a += sh[0][k];
+      b += sh[1][k];
+    }
+    partial[blockIdx.x] = make_double2(a, b);
+  }
+}
+
+// Fixed-order reduction of per-block partials -> sigma_grid = sqrt(sum d^2 / N^3)
+__global__ void k_moments_finish(const double2* __restrict__ partial, int nblk, int nrad, double invN3,
+                                 float* __restrict__ sigma_grid_out, int r0) {
+  const int r = blockIdx.x;
+  if (r >= nrad) return;
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 64) a += partial[(long long)r * nblk + k].y;
+  a = wave_sum(a);
+  if (threadIdx.x == 0) sigma_grid_out[r0 + r] = (float)sqrt(a * invN3);
+}
+
+// ------------------------------------------------ input check
+// flag = 1 if any cell is NaN or Inf (plain vector stores of the same value).
+__global__ void k_nonfinite(const float* __restrict__ x, long long n, int* __restrict__ flag) {
+  int bad = 0;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (long long)gridDim.x * blockDim.x)
+    bad |= !isfinite(x[id]);
+  if (bad) *flag = 1;
+}
+
+// ------------------------------------------------ history histogram (GPU)
+// bin = (int)((z - 49.9)/dzhistory) in double (bubblehistory_d2z.C:17-28);
+// integer counts.
+__global__ void k_history(const float* __restrict__ zre, long long total, unsigned int* __restrict__ counts,
+                          double dz) {
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const double q = ((double)zre[id] - 49.9) / dz;
+    if (q > -1.0 && q < (double)kNHistory) atomicAdd(&counts[(int)q], 1u);
+  }
+}
+
+// =================================================== FP64 scalar set-up
+// sigma^2(R) = pnorm^2 * integral over ln k of k^(3+ns) T_EH(k)^2 W(kR)^2
+// (cosmology.C:214-309) by composite 8-point Gauss-Legendre on 4096 panels
+// over [ln(1e-5/R), ln(1e5/R)]: relative error ~1e-12, below the 1e-7 the
+// reference requests from QAGS.
+struct EHd {
+  double om, ob, hh, an;
+};
+
+__device__ double transeh_d(double k, const EHd& c) {
+  const double om = c.om, ob = c.ob, hh = c.hh;
+  const double alpha = 1 - 0.328 * log(431 * om * hh * hh) * (ob / om) +
+                       0.38 * log(22.3 * om * hh * hh) * (ob / om) * (ob / om);
+  const double sound = 44.5 * log(9.83 / (om * hh * hh)) / sqrt(1 + 10 * pow(ob * hh * hh, 3. / 4.));
+  const double ks = 0.43 * k * sound;
+  const double shape = om * hh * (alpha + (1 - alpha) / (1 + ks * ks * ks * ks));
+  const double q = (k / hh) * ((2.728 / 2.7) * (2.728 / 2.7)) / shape;
+  const double lg = log(2 * 2.718281828459045235 + 1.8 * q);
+  return lg / (lg + (14.2 + 731 / (1 + 62.5 * q)) * q * q);
+}
+
+__device__ double sgm2k_d(double lk, double R, const EHd& c) {
+  const double k = exp(lk);
+  const double kR = k * R;
+  const double Pin = pow(k, 3 + c.an);
+  const double T = transeh_d(k, c);
+  const double W = 3 * (sin(kR) - kR * cos(kR)) / (kR * kR * kR);
+  return Pin * T * T * W * W;
+}
+
+__constant__ double kGLx[8] = {-0.9602898564975362, -0.7966664774136267, -0.525532409916329,
+                               -0.18343464249564978, 0.18343464249564978, 0.525532409916329,
+                               0.7966664774136267, 0.9602898564975362};
+__constant__ double kGLw[8] = {0.10122853629037669, 0.22238103445337434, 0.31370664587788705,
+                               0.36268378337836177, 0.36268378337836177, 0.31370664587788705,
+                               0.22238103445337434, 0.10122853629037669};
+
+// float radius schedule R_r = 10^(log10 R0 + r dScale) (delta2zreion_d2z.C:36-38,55)
+__device__ float radius_of(const SetupParams& p, int r) {
+#pragma clang fp contract(off)
+  const float box = p.BoxSize / p.h;
+  const float Rinitial = (box * p.Rmin) / (float)p.N;
+  const float Rfinal = p.Rmax / p.h;
+  const float dScale = (log10f(Rfinal) - log10f(Rinitial)) / (float)(p.Nscales - 1);
+  const float e = log10f(Rinitial) + (float)r * dScale;
+  return (float)pow(10.0, (double)e);
+}
+
+// R of cdmehint from a float mass: pow(3*mass/fourpi/dens, 1/3) (cosmology.C:221-231)
+__device__ double cdm_radius(const SetupParams& p, float mass) {
+#pragma clang fp contract(off)
+  const float rhoc = 2.775e11f;
+  const float pi = 3.141592654f;
+  const float fourpi = 4 * pi;
+  const double dens = (double)(((p.Omegam * rhoc) * p.h) * p.h);
+  const float num = (3.0f * mass) / fourpi;
+  return pow((double)num / dens, 1. / 3.);
+}
+
+// One block per integral: j = 0 sigma8 normalisation (R = 8/h), j = 1 the
+// minimum mass, j >= 2 radius j-2.
+__global__ void __launch_bounds__(256) k_sigma_integrals(SetupParams p, double* __restrict__ integ,
+                                                         float* __restrict__ Rout, double* __restrict__ Rint) {
+  const int j = blockIdx.x;
+  double R;
+  if (j == 0) {
+    R = (double)(8.0f / p.h);
+  } else if (j == 1) {
+    R = cdm_radius(p, p.M_min);
+  } else {
+    const float Rr = radius_of(p, j - 2);
+    const float rho0 = (float)(2.775e11 * (double)p.Omegam * (double)p.h * (double)p.h);
+    const double Rd = (double)Rr;
+    const float mass = (float)(4 * M_PI / 3 * (double)rho0 * Rd * Rd * Rd);
+    R = cdm_radius(p, mass);
+    if (threadIdx.x == 0) Rout[j - 2] = Rr;
+  }
+  if (threadIdx.x == 0) Rint[j] = R;
+  const EHd c{(double)p.Omegam, (double)p.Omegab, (double)p.h, (double)p.ns};
+  const double a = log(1e-5 / R), b = log(1e5 / R);
+  constexpr int NP = 4096;
+  const double hw = (b - a) / NP * 0.5;
+  double acc = 0.0;
+  for (int pnl = threadIdx.x; pnl < NP; pnl += blockDim.x) {
+    const double mid = a + hw * (2 * pnl + 1);
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += kGLw[q] * sgm2k_d(mid + hw * kGLx[q], R, c);
+    acc += s * hw;
+  }
+  __shared__ double sh[4];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    integ[j] = s;
+  }
+}
+
+// growth factor D(z), float Carroll-Press-Turner fit (cosmology.C:174-212)
+__device__ float growth_f(float z, float omegam, float omegal, float w) {
+#pragma clang fp contract(off)
+  const float x = 1 + z;
+  const float x2 = x * x;
+  const float x3 = x * x * x;
+  const float x3w = powf(x3, w);
+  const float curv = (1 - omegam) - omegal;
+  const float omega = omegam * x3 / ((omegam * x3 + curv * x2) + omegal);
+  const float lambda = omegal * x3 * x3w / ((omegam * x3 + curv * x2) + omegal * x3 * x3w);
+  float g = (float)(2.5 * (double)omega /
+                    (pow((double)omega, 4. / 7.) - (double)lambda +
+                     (double)((1 + omega / 2) * (1 + lambda / 70))));
+  float g0 = (float)(2.5 * (double)omegam /
+                     (pow((double)omegam, 4. / 7.) - (double)omegal +
+                      (double)((1 + omegam / 2) * (1 + omegal / 70))));
+  const double wd = (double)w;
+  const double a1 = 0.255 + 0.305 * wd + 0.0027 / wd;
+  const double a2 = 0.366 + 0.266 * wd - 0.0700 / wd;
+  const float t = (float)(-a1 * (double)(1 - omega) - a2 * (double)logf(omega));
+  const float t0 = (float)(-a1 * (double)(1 - omegam) - a2 * (double)logf(omegam));
+  g *= powf(-w, t);
+  g0 *= powf(-w, t0);
+  return (g / x) / g0;
+}
+
+// Brent's method on erfc(t) - a over [0,100] until the bracket passes
+// |hi - lo| < 1e-4 * min(|lo|,|hi|); returns the lower end (math.C:23-61).
+__device__ double inverse_erfc_lo(double a) {
+  const double eps = 2.220446049250313e-16;
+  double x_lo = 0.0, x_hi = 100.0;
+  double A = x_lo, B = x_hi;
+  double fa = erfc(A) - a, fb = erfc(B) - a;
+  double Cc = B, fc = fb;
+  double d = B - A, e = B - A;
+  double lo = x_lo, hi = x_hi;
+  for (int it = 0; it < 100; ++it) {
+    bool ac_equal = false;
+    if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) {
+      ac_equal = true;
+      Cc = A;
+      fc = fa;
+      d = e = B - A;
+    }
+    if (fabs(fc) < fabs(fb)) {
+      ac_equal = true;
+      A = B;
+      B = Cc;
+      Cc = A;
+      fa = fb;
+      fb = fc;
+      fc = fa;
+    }
+    const double tol = 0.5 * eps * fabs(B);
+    const double m = 0.5 * (Cc - B);
+    if (fb == 0) return B;
+    if (fabs(m) <= tol) return (B < Cc) ? B : Cc;
+    if (fabs(e) < tol || fabs(fa) <= fabs(fb)) {
+      d = e = m;
+    } else {
+      double pp, qq;
+      const double s = fb / fa;
+      if (ac_equal) {
+        pp = 2 * m * s;
+        qq = 1 - s;
+      } else {
+        double q = fa / fc, r = fb / fc;
+        pp = s * (2 * m * q * (q - r) - (B - A) * (r - 1));
+        qq = (q - 1) * (r - 1) * (s - 1);
+      }
+      if (pp > 0)
+        qq = -qq;
+      else
+        pp = -pp;
+      if (2 * pp < fmin(3 * m * qq - fabs(tol * qq), fabs(e * qq))) {
+        e = d;
+        d = pp / qq;
+      } else {
+        d = e = m;
+      }
+    }
+    A = B;
+    fa = fb;
+    B += (fabs(d) > tol) ? d : (m > 0 ? tol : -tol);
+    fb = erfc(B) - a;
+    if ((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) Cc = A;
+    lo = (B < Cc) ? B : Cc;
+    hi = (B < Cc) ? Cc : B;
+    double min_abs = 0.0;
+    if ((lo > 0 && hi > 0) || (lo < 0 && hi < 0)) min_abs = fmin(fabs(lo), fabs(hi));
+    if (fabs(hi - lo) < 1e-4 * min_abs) return lo;
+  }
+  return lo;
+}
+
+// Scalars: pnorm, erfizeta, sigma_min, D(zInit), D(20), A and the per-radius
+// barrier constants (delta2zreion_d2z.C:17-26,97-100,206-216; main_d2z.C:57-64).
+__global__ void k_setup_finish(SetupParams p, const double* __restrict__ integ, const float* __restrict__ Rr,
+                               SetupScalars* __restrict__ sc, RadiusConst* __restrict__ rc,
+                               float* __restrict__ sigma_out) {
+#pragma clang fp contract(off)
+  __shared__ float s_pnorm, s_smin, s_erfz, s_A;
+  if (threadIdx.x == 0) {
+    const float pnorm = (float)((double)p.sigma8 / sqrt(integ[0]));
+    const double pn = (double)pnorm;
+    const float sgm2min = (float)(integ[1] * pn * pn);
+    const float smin = sqrtf(sgm2min);
+    const float x = (float)(1.0 / (double)p.zeta);
+    const float erfz = (float)inverse_erfc_lo((double)x);
+    const float DInit = growth_f(p.zInit, p.Omegam, p.Omegal, p.w);
+    const float zHigh = 20.0f;
+    const float DHigh = growth_f(zHigh, p.Omegam, p.Omegal, p.w);
+    const float A = (1 + zHigh) * DHigh / 1.686f;
+    sc->pnorm = pnorm;
+    sc->sigma_min = smin;
+    sc->erfizeta = erfz;
+    sc->DInit = DInit;
+    sc->DHigh = DHigh;
+    sc->A = A;
+    s_pnorm = pnorm;
+    s_smin = smin;
+    s_erfz = erfz;
+    s_A = A;
+  }
+  __syncthreads();
+  const double pn = (double)s_pnorm;
+  for (int r = threadIdx.x; r < p.Nscales; r += blockDim.x) {
+    const float sgm2 = (float)(integ[2 + r] * pn * pn);
+    const float sigma = sqrtf(sgm2);
+    const float dd = s_smin * s_smin - sigma * sigma;
+    RadiusConst c;
+    c.R = Rr[r];
+    c.sigma = sigma;
+    c.A = (double)s_A;
+    c.B = sqrt(2. * (double)dd) * (double)s_erfz;
+    rc[r] = c;
+    sigma_out[r] = sigma;
+  }
+}
+
+// ===================================================== fused path kernels
+// First inverse pass (x axis) with the filter fused: one tile of the resident
+// half spectrum is loaded once and reused for nb consecutive radii; each
+// radius multiplies by W(|k| R_r)/N^3, transforms and writes its own
+// intermediate buffer T_r = T + r_local * N*N*N/2.
+template <int L, int C>
+__global__ void __launch_bounds__(Plan<L>::T* C)
+    k_fx_filter(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+                float2* __restrict__ Tbuf, int nColBlocks, float dk, float invN3,
+                const RadiusConst* __restrict__ rc, int r0, int nb, const float2* __restrict__ tw) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+  constexpr int N = L;
+  constexpr long long Nh = N / 2;
+  __shared__ float2 smem[L * C];
+  const int t = threadIdx.x;
+  const int c = t % C, i = t / C;
+  const long long blk = xcd_tile(blockIdx.x, gridDim.x);
+  const int cb = (int)(blk % nColBlocks);
+  const int ky = (int)(blk / nColBlocks);
+  const int kz = cb * C + c;
+  const long long sA = (long long)N * Nh;
+  const long long off = (long long)ky * Nh + kz;
+  const bool packed = (kz == 0);
+  LdsPencil lds{smem + c, C};
+  float2 m[E];
+  int msq[E];
+  const int fy = fsq(ky, N);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int kx = i + e * T;
+    msq[e] = fsq(kx, N) + fy;
+    m[e] = packed ? D0[(long long)kx * N + ky] : S[off + (long long)kx * sA];
+  }
+  for (int rl = 0; rl < nb; ++rl) {
+    const float dkR = dk * rc[r0 + rl].R;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (!packed) {
+        v[e] = cscale(m[e], filter_coef(msq[e] + kz * kz, dkR, invN3));
+      } else {
+        const float w0 = filter_coef(msq[e], dkR, invN3);
+        const float wn = filter_coef(msq[e] + (int)(Nh * Nh), dkR, invN3);
+        const float2 b = DN[(long long)(i + e * T) * N + ky];
+        v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
+      }
+    }
+    pencil_fft<L, +1>(v, i, tw, lds);
+    float2* dst = Tbuf + (long long)rl * N * N * Nh + off;
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[(long long)(i + e * T) * sA] = v[e];
+  }
+}
+
+// Last inverse pass (z axis, c2r) with the barrier fused: for each row the nb
+// intermediates are transformed in turn, d2z() is evaluated per cell, the
+// maximum over the batch is kept in registers (first radius wins ties) and
+// zreion is read and written once.  Sum of delta_R^2 per radius is kept in
+// FP64 per thread and reduced per block (sigma_grid).
+constexpr int kFusedMaxRadii = 8;
+
+template <int L>
+__global__ void __launch_bounds__(256)
+    k_fz_barrier(const float2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub,
+                 long long nrows, const RadiusConst* __restrict__ rc, int r0, int nb,
+                 double* __restrict__ partial, const float2* __restrict__ tw_half,
+                 const float2* __restrict__ tw_full) {
+  using P = Plan<L>;
+  constexpr int T = P::T, E = P::E;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  __shared__ float2 smem[ROWS * L];
+  __shared__ double red[kFusedMaxRadii][4];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const long long plane = (long long)L * nrows;  // complex elements per intermediate
+  LdsPencil lds{smem + rr * L, 1};
+  double s2[kFusedMaxRadii];
+#pragma unroll
+  for (int q = 0; q < kFusedMaxRadii; ++q) s2[q] = 0.0;
+  for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
+    const long long row = base + rr;
+    const bool valid = row < nrows;
+    float2 best[E];
+    float bestR[E], bestR1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      best[e] = make_float2(-INFINITY, -INFINITY);
+      bestR[e] = 0.f;
+      bestR1[e] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < kFusedMaxRadii; ++q) {
+      if (q < nb) {
+        const float2* rp = Tbuf + (long long)q * plane + (valid ? row : 0) * L;
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = rp[i + e * T];
+        c2r_preprocess<L>(v, rp, i, tw_full);
+        pencil_fft<L, +1>(v, i, tw_half, lds);
+        const RadiusConst c = rc[r0 + q];
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const float z0 = barrier_z(v[e].x, c.A, c.B);
+          const float z1 = barrier_z(v[e].y, c.A, c.B);
+          acc += (double)v[e].x * (double)v[e].x + (double)v[e].y * (double)v[e].y;
+          if (z0 > best[e].x) {
+            best[e].x = z0;
+            bestR[e] = c.R;
+          }
+          if (z1 > best[e].y) {
+            best[e].y = z1;
+            bestR1[e] = c.R;
+          }
+        }
+        if (valid) s2[q] += acc;
+      }
+    }
+    if (valid) {
+      float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
+      float2* bp = rbub ? reinterpret_cast<float2*>(rbub + row * (2 * L)) : nullptr;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int n = i + e * T;
+        float2 old = zp[n];
+        const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
+        if (u0 || u1) {
+          float2 nz = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
+          zp[n] = nz;
+          if (bp) {
+            float2 ob = bp[n];
+            bp[n] = make_float2(u0 ? bestR[e] : ob.x, u1 ? bestR1[e] : ob.y);
+          }
+        }
+      }
+    }
+  }
+  // block reduction of the per-radius sums (fixed order -> deterministic)
+  const int w = t >> 6, l = t & 63;
+#pragma unroll
+  for (int q = 0; q < kFusedMaxRadii; ++q) {
+    if (q < nb) {
+      double s = wave_sum(s2[q]);
+      if (l == 0) red[q][w] = s;
+    }
+  }
+  __syncthreads();
+  if (t < nb) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[t][k];
+    partial[(long long)(r0 + t) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// zero fill (graph-safe replacement for a memset)
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (long long)gridDim.x * blockDim.x)
+    p[id] = v;
+}
+
+// sum of the per-block partials of one radius -> sigma_grid
+__global__ void k_sigma_finish(const double* __restrict__ partial, int nblk, int nrad, double invN3,
+                               float* __restrict__ sigma_grid_out) {
+  const int r = blockIdx.x;
+  if (r >= nrad) return;
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 64) a += partial[(long long)r * nblk + k];
+  a = wave_sum(a);
+  if (threadIdx.x == 0) sigma_grid_out[r] = (float)sqrt(a * invN3);
+}
+
+// ============================================================ launchers
+static inline unsigned grid1(long long n, int bs, long long cap = 1 << 20) {
+  long long g = (n + bs - 1) / bs;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+hipError_t launch_twiddles(float2* tw, int L, hipStream_t s) {
+  hipLaunchKernelGGL(k_twiddles, dim3((L + 255) / 256), dim3(256), 0, s, tw, L);
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t fwd_z_impl(const float* in, long long in_row_stride, float2* out, long long nrows, float dinit,
+                             const float2* tw_half, const float2* tw_full, hipStream_t s) {
+  constexpr int T = Plan<L>::T;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  hipLaunchKernelGGL(k_fwd_z_r2c<L>, dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(T * ROWS), 0, s, in,
+                     in_row_stride, out, nrows, dinit, tw_half, tw_full);
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t inv_z_impl(float2* data, long long nrows, const float2* tw_half, const float2* tw_full,
+                             hipStream_t s) {
+  constexpr int T = Plan<L>::T;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  hipLaunchKernelGGL(k_inv_z_c2r<L>, dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(T * ROWS), 0, s, data,
+                     nrows, tw_half, tw_full);
+  return hipGetLastError();
+}
+
+template <int L, int DIR>
+static hipError_t strided_impl(float2* data, long long sA, long long sO, long long nOuter, int ncols,
+                               const float2* tw, hipStream_t s) {
+  constexpr int C0 = (L >= 2048) ? 8 : 16;
+  if (ncols % C0 == 0) {
+    const int ncb = ncols / C0;
+    hipLaunchKernelGGL((k_strided<L, C0, DIR>), dim3((unsigned)(nOuter * ncb)), dim3(Plan<L>::T * C0), 0, s,
+                       data, sA, sO, ncb, tw);
+  } else if (ncols % 8 == 0) {
+    const int ncb = ncols / 8;
+    hipLaunchKernelGGL((k_strided<L, 8, DIR>), dim3((unsigned)(nOuter * ncb)), dim3(Plan<L>::T * 8), 0, s, data,
+                       sA, sO, ncb, tw);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#define PRC_FOR_EACH_HALF(X) X(8) X(16) X(32) X(64) X(96) X(128) X(256) X(512) X(1024)
+#define PRC_FOR_EACH_FULL(X) X(16) X(32) X(64) X(128) X(192) X(256) X(512) X(1024) X(2048)
+
+hipError_t launch_fwd_z(int Nh, const float* in, long long in_row_stride, float2* out, long long nrows,
+                        float dinit, const float2* tw_half, const float2* tw_full, hipStream_t s) {
+  switch (Nh) {
+#define X(LL) \
+  case LL:    \
+    return fwd_z_impl<LL>(in, in_row_stride, out, nrows, dinit, tw_half, tw_full, s);
+    PRC_FOR_EACH_HALF(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_inv_z(int Nh, float2* data, long long nrows, const float2* tw_half, const float2* tw_full,
+                        hipStream_t s) {
+  switch (Nh) {
+#define X(LL) \
+  case LL:    \
+    return inv_z_impl<LL>(data, nrows, tw_half, tw_full, s);
+    PRC_FOR_EACH_HALF(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_strided(int L, int dir, float2* data, long long sA, long long sO, long long nOuter, int ncols,
+                          const float2* tw, hipStream_t s) {
+  switch (L) {
+#define X(LL)                                                                   \
+  case LL:                                                                      \
+    return dir > 0 ? strided_impl<LL, +1>(data, sA, sO, nOuter, ncols, tw, s) \
+                   : strided_impl<LL, -1>(data, sA, sO, nOuter, ncols, tw, s);
+    PRC_FOR_EACH_FULL(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_unpack(const float2* S, float2* D0, float2* DN, int N, hipStream_t s) {
+  const long long n = (long long)N * N;
+  hipLaunchKernelGGL(k_unpack_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, D0, DN, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const float2* S, const float2* D0, const float2* DN, float2* out, int N, float dkR,
+                         float invN3, hipStream_t s) {
+  const long long n = (long long)N * N * (N / 2);
+  hipLaunchKernelGGL(k_filter, dim3(grid1(n, 256)), dim3(256), 0, s, S, D0, DN, out, N, dkR, invN3);
+  return hipGetLastError();
+}
+
+hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long total, const RadiusConst* rc, int r,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_barrier, dim3(grid1(total, 256)), dim3(256), 0, s, dR, zre, rbub, total, rc, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_moments(const float* x, long long total, double2* partial, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(256), 0, s, x, total, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_moments_finish(const double2* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
+                                 int r0, hipStream_t s) {
+  hipLaunchKernelGGL(k_moments_finish, dim3(nrad), dim3(64), 0, s, partial, nblk, nrad, invN3, sigma_grid_out,
+                     r0);
+  return hipGetLastError();
+}
+
+hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, hipStream_t s) {
+  hipLaunchKernelGGL(k_history, dim3(grid1(total, 256, 8192)), dim3(256), 0, s, zre, total, counts, dz);
+  return hipGetLastError();
+}
+
+hipError_t launch_setup(const SetupParams& p, double* integ, float* Rr, double* Rint, SetupScalars* sc,
+                        RadiusConst* rc, float* sigma_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sigma_integrals, dim3(p.Nscales + 2), dim3(256), 0, s, p, integ, Rr, Rint);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(64), 0, s, p, integ, Rr, sc, rc, sigma_out);
+  return hipGetLastError();
+}
+
+template <int L>
+static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, float2* Tbuf, int N, float dk,
+                          float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+  constexpr int C = (L >= 512) ? 8 : ((L / 2) >= 16 ? 16 : 8);
+  const int Nh = N / 2;
+  if (Nh % C != 0) return hipErrorInvalidValue;
+  const int ncb = Nh / C;
+  hipLaunchKernelGGL((k_fx_filter<L, C>), dim3((unsigned)(N * ncb)), dim3(Plan<L>::T * C), 0, s, S, D0, DN, Tbuf,
+                     ncb, dk, invN3, rc, r0, nb, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
+                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+  switch (N) {
+#define X(LL) \
+  case LL:    \
+    return fx_impl<LL>(S, D0, DN, Tbuf, N, dk, invN3, rc, r0, nb, tw, s);
+    PRC_FOR_EACH_FULL(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int L>
+static hipError_t fz_impl(const float2* Tbuf, float* zre, float* rbub, long long nrows, const RadiusConst* rc,
+                          int r0, int nb, double* partial, int nblk, const float2* tw_half, const float2* tw_full,
+                          hipStream_t s) {
+  constexpr int T = Plan<L>::T;
+  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
+  hipLaunchKernelGGL(k_fz_barrier<L>, dim3(nblk), dim3(T * ROWS), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
+                     partial, tw_half, tw_full);
+  return hipGetLastError();
+}
+
+int fz_blocks(int Nh, long long nrows) {
+  int T = 1;
+  switch (Nh) {
+#define X(LL)          \
+  case LL:             \
+    T = Plan<LL>::T;   \
+    break;
+    PRC_FOR_EACH_HALF(X)
+#undef X
+    default:
+      return 0;
+  }
+  const int rows = (256 / T) > 0 ? 256 / T : 1;
+  long long need = (nrows + rows - 1) / rows;
+  return (int)(need < 4096 ? need : 4096);
+}
+
+hipError_t launch_fz(int Nh, const float2* Tbuf, float* zre, float* rbub, long long nrows, const RadiusConst* rc,
+                     int r0, int nb, double* partial, int nblk, const float2* tw_half, const float2* tw_full,
+                     hipStream_t s) {
+  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+  switch (Nh) {
+#define X(LL) \
+  case LL:    \
+    return fz_impl<LL>(Tbuf, zre, rbub, nrows, rc, r0, nb, partial, nblk, tw_half, tw_full, s);
+    PRC_FOR_EACH_HALF(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(grid1(n, 256, 16384)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_sigma_finish, dim3(nrad), dim3(64), 0, s, partial, nblk, nrad, invN3, sigma_grid_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_nonfinite(const float* x, long long n, int* flag, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nonfinite, dim3(grid1(n, 256, 8192)), dim3(256), 0, s, x, n, flag);
+  return hipGetLastError();
+}
+
+}  // namespace prc

@@ -1,0 +1,289 @@
+// Device building blocks for the pencil FFT passes (gfx950, wave64).
+//
+// A pencil of length L is owned by T = L/E threads; thread i holds the E
+// elements x[i + e*T], e = 0..E-1, in registers.  Each pass runs a
+// self-sorting (Stockham) sequence of radix stages; between stages the
+// pencil is exchanged through LDS.  With this register layout the first
+// stage reads x[i + e*T] and the last stage produces y[i + e*T] in natural
+// order, so global loads and stores are coalesced along whichever memory axis
+// the caller maps consecutive lanes to.
+//
+// Stage (p, R), p = product of the radices already applied:
+//   butterfly b = i + m*T (m < E/R) takes x[b + r*L/R] = v[m + r*E/R],
+//   multiplies by w_{pR}^{r*(b mod p)}, applies the R-point DFT and writes
+//   y[(b/p)*p*R + (b mod p) + r*p].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace prc {
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// a * (DIR * i)
+template <int DIR>
+__device__ __forceinline__ float2 mul_di(float2 a) {
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ------------------------------------------------- compile-time trigonometry
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int n = 1; n < 14; ++n) { term *= -x2 / ((2 * n) * (2 * n + 1)); sum += term; }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) { term *= -x2 / ((2 * n - 1) * (2 * n)); sum += term; }
+  return sum;
+}
+// cos(2 pi m / R) and sin(2 pi m / R) for integers, reduced by octant so the
+// series argument stays within pi/4.
+constexpr void twiddle_cs(long m, long R, double& c, double& s) {
+  m %= R;
+  if (m < 0) m += R;
+  // angle = 2 pi m / R; octant o = floor(8 m / R)
+  long o = (8 * m) / R;
+  // residual angle measured from the octant's reference axis
+  double frac = (8.0 * (double)m - (double)(o * R)) / (double)R;  // in [0,1)
+  double a = frac * (kPi / 4);                                    // in [0, pi/4)
+  double ca = taylor_cos(a), sa = taylor_sin(a);
+  double cb = taylor_cos(kPi / 4 - a), sb = taylor_sin(kPi / 4 - a);
+  switch (o) {
+    case 0: c = ca; s = sa; break;
+    case 1: c = sb; s = cb; break;
+    case 2: c = -sa; s = ca; break;
+    case 3: c = -cb; s = sb; break;
+    case 4: c = -ca; s = -sa; break;
+    case 5: c = -sb; s = -cb; break;
+    case 6: c = sa; s = -ca; break;
+    default: c = cb; s = -sb; break;
+  }
+}
+constexpr double tw_c(long m, long R) { double c = 0, s = 0; twiddle_cs(m, R, c, s); return c; }
+constexpr double tw_s(long m, long R) { double c = 0, s = 0; twiddle_cs(m, R, c, s); return s; }
+
+// v * exp(DIR * 2 pi i m / R) with m, R compile-time constants
+template <int DIR, long M, long R>
+__device__ __forceinline__ float2 twc(float2 v) {
+  constexpr long m = ((M % R) + R) % R;
+  if constexpr (m == 0) {
+    return v;
+  } else if constexpr (4 * m == R) {
+    return mul_di<DIR>(v);
+  } else if constexpr (2 * m == R) {
+    return make_float2(-v.x, -v.y);
+  } else if constexpr (4 * m == 3 * R) {
+    return mul_di<-DIR>(v);
+  } else {
+    constexpr float c = (float)tw_c(m, R);
+    constexpr float s = (float)(DIR * tw_s(m, R));
+    return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+}
+
+// ------------------------------------------------ in-register small DFTs
+// y[k] = sum_n x[n] exp(DIR 2 pi i n k / R), in place, natural order.
+template <int R, int DIR>
+struct DFT;
+
+template <int DIR>
+struct DFT<1, DIR> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR>
+struct DFT<2, DIR> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int DIR>
+struct DFT<3, DIR> {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float kS = 0.86602540378443864676f;  // sqrt(3)/2
+    float2 s = cadd(v[1], v[2]);
+    float2 d = csub(v[1], v[2]);
+    float2 y0 = cadd(v[0], s);
+    float2 t1 = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+    float2 t2 = cscale(mul_di<DIR>(d), kS);
+    v[0] = y0;
+    v[1] = cadd(t1, t2);
+    v[2] = csub(t1, t2);
+  }
+};
+
+template <int DIR>
+struct DFT<4, DIR> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]);
+    float2 t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]);
+    float2 t3 = mul_di<DIR>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+constexpr int first_factor(int R) { return (R % 4 == 0 && R != 4) ? 4 : (R % 2 == 0 ? 2 : 3); }
+
+// Composite R = R1 * R2: n = n1 + R1 n2, k = k2 + R2 k1.
+template <int R, int DIR>
+struct DFT {
+  static constexpr int R1 = first_factor(R);
+  static constexpr int R2 = R / R1;
+  template <int N1, int K2>
+  __device__ __forceinline__ static float2 tw(float2 v) { return twc<DIR, (long)N1 * K2, R>(v); }
+
+  template <int N1>
+  __device__ __forceinline__ static void inner(const float2* v, float2 (&a)[R1][R2]) {
+    float2 t[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) t[n2] = v[N1 + R1 * n2];
+    DFT<R2, DIR>::run(t);
+    twiddle_row<N1, 0>(t, a);
+  }
+  template <int N1, int K2>
+  __device__ __forceinline__ static void twiddle_row(const float2* t, float2 (&a)[R1][R2]) {
+    if constexpr (K2 < R2) {
+      a[N1][K2] = tw<N1, K2>(t[K2]);
+      twiddle_row<N1, K2 + 1>(t, a);
+    }
+  }
+  template <int N1>
+  __device__ __forceinline__ static void all_inner(const float2* v, float2 (&a)[R1][R2]) {
+    if constexpr (N1 < R1) {
+      inner<N1>(v, a);
+      all_inner<N1 + 1>(v, a);
+    }
+  }
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a[R1][R2];
+    all_inner<0>(v, a);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      float2 t[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) t[n1] = a[n1][k2];
+      DFT<R1, DIR>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k2 + R2 * k1] = t[k1];
+    }
+  }
+};
+
+// --------------------------------------------------------------- plans
+// Per transform length: E elements per thread and the radix sequence.
+template <int L>
+struct Plan;
+#define PRC_PLAN(LEN, EE, ...)                                    \
+  template <>                                                     \
+  struct Plan<LEN> {                                              \
+    static constexpr int L = LEN, E = EE, T = LEN / EE;           \
+    static constexpr int radices[] = {__VA_ARGS__};               \
+    static constexpr int S = sizeof(radices) / sizeof(int);       \
+  };
+PRC_PLAN(8, 8, 8)
+PRC_PLAN(16, 16, 16)
+PRC_PLAN(32, 8, 8, 4)
+PRC_PLAN(64, 8, 8, 8)
+PRC_PLAN(96, 24, 3, 8, 4)
+PRC_PLAN(128, 16, 16, 8)
+PRC_PLAN(192, 24, 3, 8, 8)
+PRC_PLAN(256, 16, 16, 16)
+PRC_PLAN(512, 8, 8, 8, 8)
+PRC_PLAN(1024, 16, 16, 16, 4)
+PRC_PLAN(2048, 16, 16, 16, 8)
+#undef PRC_PLAN
+
+template <int L, int S>
+constexpr int plan_p() {  // product of the first S radices
+  int p = 1;
+  for (int s = 0; s < S; ++s) p *= Plan<L>::radices[s];
+  return p;
+}
+
+// LDS view of one pencil: element a at base[a * sa]
+struct LdsPencil {
+  float2* base;
+  int sa;
+  __device__ __forceinline__ float2& operator[](int a) const { return base[a * sa]; }
+};
+
+// One Stockham stage on registers v (thread index i within the pencil).
+// If LAST, outputs stay in registers (natural order, v[e] = y[i + e*T]);
+// otherwise they are written to LDS at their Stockham positions.
+template <int L, int STAGE, int DIR, bool LAST>
+__device__ __forceinline__ void stockham_stage(float2 (&v)[Plan<L>::E], int i, const float2* __restrict__ tw,
+                                               LdsPencil lds) {
+  using P = Plan<L>;
+  constexpr int E = P::E, T = P::T;
+  constexpr int R = P::radices[STAGE];
+  constexpr int p = plan_p<L, STAGE>();
+  constexpr int NB = E / R;      // butterflies per thread
+  constexpr int TWS = L / (p * R);  // twiddle table stride
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[m + r * NB];
+    const int b = i + m * T;
+    const int k = (p == 1) ? 0 : (b % p);
+    if constexpr (p > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * k * TWS];
+        if (DIR < 0) w.y = -w.y;
+        u[r] = cmul(u[r], w);
+      }
+    }
+    DFT<R, DIR>::run(u);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m + r * NB] = u[r];
+    } else {
+      const int j = (b - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[j + r * p] = u[r];
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void lds_gather(float2 (&v)[Plan<L>::E], int i, LdsPencil lds) {
+  using P = Plan<L>;
+#pragma unroll
+  for (int e = 0; e < P::E; ++e) v[e] = lds[i + e * P::T];
+}
+
+// Full pencil transform: registers in natural layout -> registers in natural
+// layout.  All threads of the block must call it (it contains barriers).
+template <int L, int DIR, int STAGE = 0>
+__device__ __forceinline__ void pencil_fft(float2 (&v)[Plan<L>::E], int i, const float2* __restrict__ tw,
+                                           LdsPencil lds) {
+  using P = Plan<L>;
+  if constexpr (STAGE == P::S - 1) {
+    stockham_stage<L, STAGE, DIR, true>(v, i, tw, lds);
+  } else {
+    stockham_stage<L, STAGE, DIR, false>(v, i, tw, lds);
+    __syncthreads();
+    lds_gather<L>(v, i, lds);
+    __syncthreads();
+    pencil_fft<L, DIR, STAGE + 1>(v, i, tw, lds);
+  }
+}
+
+}  // namespace prc

@@ -1,0 +1,618 @@
+// C-ABI implementation of libprc_d2z: context, device buffers, the radius
+// loop orchestration and per-kernel event timing.
+//
+// Reference call sequence replaced (marcelo-alvarez/pr-calc):
+//   main_d2z.C:32-48  fftwf_mpi_local_size_3d, AllocateArrays, plans  -> prc_d2z_create
+//   main_d2z.C:52-59  ReadDeltaFile destination, delta /= DInit       -> prc_d2z_set_delta + run
+//   main_d2z.C:67     fftwf_execute(plan)                             -> run: forward passes
+//   main_d2z.C:71     delta2zreion()                                  -> run: radius loop
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/prc_d2z.h"
+#include "d2z_kernels.h"
+
+using namespace prc;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct KernelTimer {
+  // event pairs recorded around launches while timing is on
+  struct Rec {
+    int cat;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::string> cats;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  int cat(const char* name) {
+    for (size_t i = 0; i < cats.size(); ++i)
+      if (cats[i] == name) return (int)i;
+    cats.emplace_back(name);
+    return (int)cats.size() - 1;
+  }
+  void reset() {
+    recs.clear();
+    used = 0;
+  }
+  void destroy() {
+    for (auto e : pool) hipEventDestroy(e);
+    pool.clear();
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
+    t0 = t1 = nullptr;
+  }
+};
+
+constexpr int kDefaultRadiiPerLaunch = 8;
+constexpr int kMomentBlocks = 1024;
+
+bool supported_n(int N) {
+  switch (N) {
+    case 16: case 32: case 64: case 128: case 192: case 256: case 512: case 1024: case 2048:
+      return true;
+    default:
+      return false;
+  }
+}
+
+}  // namespace
+
+struct prc_d2z_ctx {
+  prc_d2z_params p{};
+  int N = 0, Nh = 0;
+  long long ncell = 0, nspec = 0;  // N^3 reals, N*N*N/2 complex
+  hipStream_t stream = nullptr;
+  float* d_delta = nullptr;
+  float2* d_S = nullptr;
+  float2* d_D0 = nullptr;
+  float2* d_DN = nullptr;
+  float2* d_T = nullptr;
+  int nbuf = 1;
+  float* d_zre = nullptr;
+  float* d_rb = nullptr;
+  float2* d_tw_full = nullptr;
+  float2* d_tw_half = nullptr;
+  double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
+  double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
+  int nblk_fz = 0;
+  float* d_sigma_grid = nullptr;
+  float* d_sigma = nullptr;
+  RadiusConst* d_rc = nullptr;
+  SetupScalars* d_sc = nullptr;
+  double* d_integ = nullptr;
+  double* d_Rint = nullptr;
+  float* d_R = nullptr;
+  unsigned* d_hist = nullptr;
+  int* d_flag = nullptr;
+  SetupScalars sc{};
+  std::vector<RadiusConst> rc;
+  bool have_delta = false, have_spectrum = false, have_result = false;
+  std::string err;
+  size_t bytes = 0;
+  bool timing = false;
+  KernelTimer timer;
+  std::vector<void*> allocs;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* what) {
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return PRC_D2Z_ERR_HIP;
+  }
+  template <class T>
+  hipError_t alloc(T** ptr, size_t bytes_) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes_);
+    if (e == hipSuccess) {
+      allocs.push_back(q);
+      bytes += bytes_;
+      *ptr = (T*)q;
+    }
+    return e;
+  }
+};
+
+namespace {
+
+// Launch wrapper that records timing events when enabled.
+template <class F>
+hipError_t timed(prc_d2z_ctx* c, const char* name, F&& f) {
+  if (!c->timing) return f();
+  KernelTimer& t = c->timer;
+  hipEvent_t a = t.get(), b = t.get();
+  if (!a || !b) return hipErrorOutOfMemory;
+  (void)hipEventRecord(a, c->stream);
+  hipError_t e = f();
+  (void)hipEventRecord(b, c->stream);
+  t.recs.push_back({t.cat(name), a, b});
+  return e;
+}
+
+#define PRC_CHECK(ctx, expr, what)                       \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return (ctx)->hip_fail(e_, what); \
+  } while (0)
+
+int forward(prc_d2z_ctx* c) {
+  const int N = c->N, Nh = c->Nh;
+  hipStream_t s = c->stream;
+  PRC_CHECK(c, timed(c, "fwd_z_r2c", [&] {
+              return launch_fwd_z(Nh, c->d_delta, N, c->d_S, (long long)N * N, c->sc.DInit, c->d_tw_half,
+                                  c->d_tw_full, s);
+            }),
+            "forward z pass");
+  PRC_CHECK(c, timed(c, "fwd_y", [&] {
+              return launch_strided(N, -1, c->d_S, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s);
+            }),
+            "forward y pass");
+  PRC_CHECK(c, timed(c, "fwd_x", [&] {
+              return launch_strided(N, -1, c->d_S, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s);
+            }),
+            "forward x pass");
+  PRC_CHECK(c, timed(c, "unpack", [&] { return launch_unpack(c->d_S, c->d_D0, c->d_DN, N, s); }), "unpack");
+  c->have_spectrum = true;
+  return PRC_D2Z_OK;
+}
+
+float host_dk(const prc_d2z_ctx* c) {
+  // float dk = 2.*M_PI/BoxSize with BoxSize already divided by h (delta2zreion_d2z.C:41)
+  const float box = c->p.BoxSize / c->p.h;
+  return (float)(2. * M_PI / (double)box);
+}
+
+// Plain path, one radius: separate filter, three inverse passes, moments, barrier.
+int plain_radius(prc_d2z_ctx* c, int r, bool barrier) {
+  const int N = c->N, Nh = c->Nh;
+  hipStream_t s = c->stream;
+  const float dkR = host_dk(c) * c->rc[r].R;
+  const float invN3 = (float)(1.0 / ((double)N * N * N));
+  PRC_CHECK(c, timed(c, "filter", [&] {
+              return launch_filter(c->d_S, c->d_D0, c->d_DN, c->d_T, N, dkR, invN3, s);
+            }),
+            "filter");
+  PRC_CHECK(c, timed(c, "inv_x", [&] {
+              return launch_strided(N, +1, c->d_T, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s);
+            }),
+            "inverse x pass");
+  PRC_CHECK(c, timed(c, "inv_y", [&] {
+              return launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s);
+            }),
+            "inverse y pass");
+  PRC_CHECK(c, timed(c, "inv_z_c2r", [&] {
+              return launch_inv_z(Nh, c->d_T, (long long)N * N, c->d_tw_half, c->d_tw_full, s);
+            }),
+            "inverse z pass");
+  if (!barrier) return PRC_D2Z_OK;
+  const float* dR = reinterpret_cast<const float*>(c->d_T);
+  PRC_CHECK(c, timed(c, "moments", [&] {
+              return launch_moments(dR, c->ncell, c->d_mom + (long long)r * kMomentBlocks, kMomentBlocks, s);
+            }),
+            "moments");
+  PRC_CHECK(c, timed(c, "barrier", [&] {
+              return launch_barrier(dR, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->ncell, c->d_rc, r,
+                                    s);
+            }),
+            "barrier");
+  return PRC_D2Z_OK;
+}
+
+int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
+  const int N = c->N, Nh = c->Nh;
+  hipStream_t s = c->stream;
+  const float invN3 = (float)(1.0 / ((double)N * N * N));
+  PRC_CHECK(c, timed(c, "fx_filter", [&] {
+              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
+                               c->d_tw_full, s);
+            }),
+            "fused x pass");
+  PRC_CHECK(c, timed(c, "inv_y", [&] {
+              return launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, (long long)nb * N, Nh, c->d_tw_full, s);
+            }),
+            "inverse y pass");
+  PRC_CHECK(c, timed(c, "fz_barrier", [&] {
+              return launch_fz(Nh, c->d_T, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, (long long)N * N,
+                               c->d_rc, r0, nb, c->d_part, c->nblk_fz, c->d_tw_half, c->d_tw_full, s);
+            }),
+            "fused z pass");
+  return PRC_D2Z_OK;
+}
+
+int run_all(prc_d2z_ctx* c) {
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  hipStream_t s = c->stream;
+  if (c->timing) {
+    c->timer.reset();
+    if (!c->timer.t0) {
+      (void)hipEventCreate(&c->timer.t0);
+      (void)hipEventCreate(&c->timer.t1);
+    }
+    (void)hipEventRecord(c->timer.t0, s);
+  }
+  int rc = forward(c);
+  if (rc) return rc;
+  PRC_CHECK(c, launch_fill(c->d_zre, c->ncell, 0.0f, s), "zreion init");
+  if (c->p.RecordBubbleSizes) PRC_CHECK(c, launch_fill(c->d_rb, c->ncell, 0.0f, s), "rbubble init");
+  const int NS = c->p.Nscales;
+  const double invN3 = 1.0 / ((double)c->N * c->N * c->N);
+  if (c->p.mode == PRC_D2Z_PLAIN_FP32) {
+    for (int r = 0; r < NS; ++r)
+      if ((rc = plain_radius(c, r, true))) return rc;
+    PRC_CHECK(c, launch_moments_finish(c->d_mom, kMomentBlocks, NS, invN3, c->d_sigma_grid, 0, s), "sigma_grid");
+  } else {
+    for (int r0 = 0; r0 < NS; r0 += c->nbuf) {
+      const int nb = std::min(c->nbuf, NS - r0);
+      if ((rc = fused_batch(c, r0, nb))) return rc;
+    }
+    PRC_CHECK(c, launch_sigma_finish(c->d_part, c->nblk_fz, NS, invN3, c->d_sigma_grid, s), "sigma_grid");
+  }
+  if (c->timing) hipEventRecord(c->timer.t1, s);
+  c->have_result = true;
+  return PRC_D2Z_OK;
+}
+
+int copy_field_out(prc_d2z_ctx* c, const float* dsrc, float* out, int layout) {
+  const int N = c->N;
+  if (layout == PRC_D2Z_DENSE) {
+    PRC_CHECK(c, hipMemcpyAsync(out, dsrc, sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream), "copy out");
+  } else if (layout == PRC_D2Z_PADDED) {
+    PRC_CHECK(c, hipMemcpy2DAsync(out, sizeof(float) * (N + 2), dsrc, sizeof(float) * N, sizeof(float) * N,
+                                  (size_t)N * N, hipMemcpyDeviceToHost, c->stream),
+              "copy out");
+  } else {
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "unknown layout");
+  }
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "copy out");
+  if (layout == PRC_D2Z_PADDED)
+    for (long long row = 0; row < (long long)N * N; ++row) out[row * (N + 2) + N] = out[row * (N + 2) + N + 1] = 0.f;
+  return PRC_D2Z_OK;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int prc_d2z_abi_version(void) { return PRC_D2Z_ABI_VERSION; }
+
+void prc_d2z_default_params(prc_d2z_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->Omegam = 0.31f;
+  p->Omegab = 0.049f;
+  p->Omegal = 0.69f;
+  p->h = 0.68f;
+  p->ns = 0.965f;
+  p->sigma8 = 0.81f;
+  p->w = -1.0f;
+  p->BoxSize = 200.0f;
+  p->zInit = 0.0f;
+  p->Rmin = 1.0f;
+  p->N = 128;
+  p->Nscales = 32;
+  p->RecordBubbleSizes = 0;
+  p->Rmax = 256.0f;
+  p->zeta = 10.0f;
+  p->M_min = 1e8f;
+  p->mode = PRC_D2Z_FUSED_FP32;
+  p->radii_per_launch = 0;
+  p->device = 0;
+}
+
+const char* prc_d2z_last_error(const prc_d2z_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+size_t prc_d2z_device_bytes(const prc_d2z_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+void prc_d2z_destroy(prc_d2z_ctx* c) {
+  if (!c) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(c->p.device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  c->timer.destroy();
+  for (void* q : c->allocs) hipFree(q);
+  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipSetDevice(dev);
+  delete c;
+}
+
+int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
+  g_create_error.clear();
+  if (!p || !out) {
+    g_create_error = "null argument";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  *out = nullptr;
+  if (!supported_n(p->N)) {
+    g_create_error = "unsupported N=" + std::to_string(p->N) +
+                     " (supported: 16, 32, 64, 128, 192, 256, 512, 1024, 2048)";
+    return PRC_D2Z_ERR_UNSUPPORTED_N;
+  }
+  if (p->Nscales < 2) {
+    g_create_error = "Nscales must be >= 2 (the radius spacing divides by Nscales-1)";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  if (!(p->h > 0) || !(p->BoxSize > 0) || !(p->Rmin > 0) || !(p->Rmax > 0) || !(p->zeta > 1) || !(p->M_min > 0) ||
+      !(p->Omegam > 0) || !(p->sigma8 > 0)) {
+    g_create_error = "parameters out of range (need h, BoxSize, Rmin, Rmax, M_min, Omegam, sigma8 > 0 and zeta > 1)";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  if (p->mode != PRC_D2Z_PLAIN_FP32 && p->mode != PRC_D2Z_FUSED_FP32) {
+    g_create_error = "mode not available in this build";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  if (p->radii_per_launch < 0 || p->radii_per_launch > 8) {
+    g_create_error = "radii_per_launch must be in 0..8";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_error = "no HIP device";
+    return PRC_D2Z_ERR_NO_DEVICE;
+  }
+  if (p->device < 0 || p->device >= ndev) {
+    g_create_error = "device ordinal out of range";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  prc_d2z_ctx* c = new (std::nothrow) prc_d2z_ctx();
+  if (!c) {
+    g_create_error = "host allocation failed";
+    return PRC_D2Z_ERR_OOM;
+  }
+  c->p = *p;
+  c->N = p->N;
+  c->Nh = p->N / 2;
+  c->ncell = (long long)p->N * p->N * p->N;
+  c->nspec = (long long)p->N * p->N * c->Nh;
+  c->nbuf = (p->mode == PRC_D2Z_PLAIN_FP32) ? 1
+                                            : std::min(p->Nscales, p->radii_per_launch ? p->radii_per_launch
+                                                                                       : kDefaultRadiiPerLaunch);
+  auto bail = [&](int code, const std::string& m) {
+    g_create_error = m;
+    prc_d2z_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
+  // memory budget check before allocating
+  const int NS = p->Nscales;
+  c->nblk_fz = fz_blocks(c->Nh, (long long)p->N * p->N);
+  size_t need = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) +
+                sizeof(float2) * c->nspec * (1 + c->nbuf) + sizeof(float2) * 2 * (size_t)p->N * p->N;
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && need + (64u << 20) > freeb)
+    return bail(PRC_D2Z_ERR_OOM, "device memory: need " + std::to_string(need >> 20) + " MiB, free " +
+                                     std::to_string(freeb >> 20) + " MiB");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
+  hipError_t e = hipSuccess;
+#define A_(ptr, n) \
+  if (e == hipSuccess) e = c->alloc(&(ptr), (n))
+  A_(c->d_delta, sizeof(float) * c->ncell);
+  A_(c->d_S, sizeof(float2) * c->nspec);
+  A_(c->d_D0, sizeof(float2) * (size_t)p->N * p->N);
+  A_(c->d_DN, sizeof(float2) * (size_t)p->N * p->N);
+  A_(c->d_T, sizeof(float2) * c->nspec * c->nbuf);
+  A_(c->d_zre, sizeof(float) * c->ncell);
+  if (p->RecordBubbleSizes) A_(c->d_rb, sizeof(float) * c->ncell);
+  A_(c->d_tw_full, sizeof(float2) * p->N);
+  A_(c->d_tw_half, sizeof(float2) * c->Nh);
+  A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
+  A_(c->d_part, sizeof(double) * (size_t)NS * std::max(c->nblk_fz, 1));
+  A_(c->d_sigma_grid, sizeof(float) * NS);
+  A_(c->d_sigma, sizeof(float) * NS);
+  A_(c->d_rc, sizeof(RadiusConst) * NS);
+  A_(c->d_sc, sizeof(SetupScalars));
+  A_(c->d_integ, sizeof(double) * (NS + 2));
+  A_(c->d_Rint, sizeof(double) * (NS + 2));
+  A_(c->d_R, sizeof(float) * NS);
+  A_(c->d_hist, sizeof(unsigned) * kNHistory);
+  A_(c->d_flag, sizeof(int));
+#undef A_
+  if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("device allocation failed: ") + hipGetErrorString(e));
+  // twiddle tables and the FP64 scalar set-up, all on the device
+  SetupParams sp{p->Omegam, p->Omegab, p->Omegal, p->h,    p->ns,   p->sigma8, p->w, p->BoxSize,
+                 p->zInit,  p->Rmin,   p->Rmax,   p->zeta, p->M_min, p->N,      p->Nscales};
+  if ((e = launch_twiddles(c->d_tw_full, p->N, c->stream)) != hipSuccess ||
+      (e = launch_twiddles(c->d_tw_half, c->Nh, c->stream)) != hipSuccess ||
+      (e = launch_setup(sp, c->d_integ, c->d_R, c->d_Rint, c->d_sc, c->d_rc, c->d_sigma, c->stream)) != hipSuccess)
+    return bail(PRC_D2Z_ERR_HIP, std::string("set-up kernels: ") + hipGetErrorString(e));
+  c->rc.resize(NS);
+  if ((e = hipMemcpyAsync(&c->sc, c->d_sc, sizeof(SetupScalars), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(c->rc.data(), c->d_rc, sizeof(RadiusConst) * NS, hipMemcpyDeviceToHost, c->stream)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return bail(PRC_D2Z_ERR_HIP, std::string("set-up copy: ") + hipGetErrorString(e));
+  *out = c;
+  return PRC_D2Z_OK;
+}
+
+static int set_delta_impl(prc_d2z_ctx* c, const float* src, int layout, hipMemcpyKind kind) {
+  if (!c || !src) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "null field") : PRC_D2Z_ERR_BAD_ARG;
+  (void)hipSetDevice(c->p.device);
+  const int N = c->N;
+  if (layout == PRC_D2Z_DENSE) {
+    PRC_CHECK(c, hipMemcpyAsync(c->d_delta, src, sizeof(float) * c->ncell, kind, c->stream), "copy in");
+  } else if (layout == PRC_D2Z_PADDED) {
+    PRC_CHECK(c, hipMemcpy2DAsync(c->d_delta, sizeof(float) * N, src, sizeof(float) * (N + 2), sizeof(float) * N,
+                                  (size_t)N * N, kind, c->stream),
+              "copy in");
+  } else {
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "unknown layout");
+  }
+  int bad = 0;
+  PRC_CHECK(c, launch_count_nonfinite(c->d_delta, c->ncell, c->d_flag, c->stream), "input check");
+  PRC_CHECK(c, hipMemcpyAsync(&bad, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream), "input check");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "copy in");
+  c->have_delta = false;
+  c->have_spectrum = false;
+  c->have_result = false;
+  if (bad) return c->fail(PRC_D2Z_ERR_NAN_INPUT, "density field holds NaN or Inf");
+  c->have_delta = true;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_set_delta(prc_d2z_ctx* c, const float* delta, int layout) {
+  return set_delta_impl(c, delta, layout, hipMemcpyHostToDevice);
+}
+
+int prc_d2z_set_delta_device(prc_d2z_ctx* c, const float* d_delta, int layout) {
+  return set_delta_impl(c, d_delta, layout, hipMemcpyDeviceToDevice);
+}
+
+int prc_d2z_run_async(prc_d2z_ctx* c) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  (void)hipSetDevice(c->p.device);
+  return run_all(c);
+}
+
+int prc_d2z_sync(prc_d2z_ctx* c) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "synchronize");
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_run(prc_d2z_ctx* c) {
+  int rc = prc_d2z_run_async(c);
+  if (rc) return rc;
+  return prc_d2z_sync(c);
+}
+
+int prc_d2z_get_zreion(prc_d2z_ctx* c, float* out, int layout) {
+  if (!c || !out) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "null output") : PRC_D2Z_ERR_BAD_ARG;
+  if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
+  return copy_field_out(c, c->d_zre, out, layout);
+}
+
+int prc_d2z_get_rbubble(prc_d2z_ctx* c, float* out, int layout) {
+  if (!c || !out) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "null output") : PRC_D2Z_ERR_BAD_ARG;
+  if (!c->p.RecordBubbleSizes) return c->fail(PRC_D2Z_ERR_STATE, "RecordBubbleSizes is 0");
+  if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
+  return copy_field_out(c, c->d_rb, out, layout);
+}
+
+int prc_d2z_zreion_device(prc_d2z_ctx* c, const float** d) {
+  if (!c || !d) return PRC_D2Z_ERR_BAD_ARG;
+  *d = c->d_zre;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_scalars(prc_d2z_ctx* c, prc_d2z_scalars* o) {
+  if (!c || !o) return PRC_D2Z_ERR_BAD_ARG;
+  o->DInit = c->sc.DInit;
+  o->DHigh = c->sc.DHigh;
+  o->A = c->sc.A;
+  o->pnorm = c->sc.pnorm;
+  o->erfizeta = c->sc.erfizeta;
+  o->sigma_min = c->sc.sigma_min;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_radius_table(prc_d2z_ctx* c, prc_d2z_radius_row* rows, int n) {
+  if (!c || !rows || n < c->p.Nscales) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "table too small") : PRC_D2Z_ERR_BAD_ARG;
+  std::vector<float> sg(c->p.Nscales, 0.f);
+  if (c->have_result) {
+    PRC_CHECK(c, hipMemcpyAsync(sg.data(), c->d_sigma_grid, sizeof(float) * sg.size(), hipMemcpyDeviceToHost,
+                                c->stream),
+              "table copy");
+    PRC_CHECK(c, hipStreamSynchronize(c->stream), "table copy");
+  }
+  for (int r = 0; r < c->p.Nscales; ++r) {
+    rows[r].R_mpch = c->rc[r].R * c->p.h;
+    rows[r].sigma = c->rc[r].sigma;
+    rows[r].sigma_grid = sg[r];
+  }
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_history(prc_d2z_ctx* c, uint32_t* counts) {
+  if (!c || !counts) return PRC_D2Z_ERR_BAD_ARG;
+  if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
+  (void)hipSetDevice(c->p.device);
+  // dzhistory = (HISTORY_FINAL - HISTORY_INITIAL)/(NHISTORY-1) stored as float
+  const float dzf = (float)((0.01 - 49.9) / (kNHistory - 1));
+  PRC_CHECK(c, hipMemsetAsync(c->d_hist, 0, sizeof(unsigned) * kNHistory, c->stream), "history");
+  PRC_CHECK(c, launch_history(c->d_zre, c->ncell, c->d_hist, (double)dzf, c->stream), "history");
+  PRC_CHECK(c, hipMemcpyAsync(counts, c->d_hist, sizeof(unsigned) * kNHistory, hipMemcpyDeviceToHost, c->stream),
+            "history");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "history");
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
+  if (!c || !out) return PRC_D2Z_ERR_BAD_ARG;
+  if (r < 0 || r >= c->p.Nscales) return c->fail(PRC_D2Z_ERR_BAD_ARG, "radius index out of range");
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  (void)hipSetDevice(c->p.device);
+  const bool was_timing = c->timing;
+  c->timing = false;
+  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c);
+  if (!rc) {
+    if (c->p.mode == PRC_D2Z_PLAIN_FP32) {
+      rc = plain_radius(c, r, false);
+    } else {
+      const int N = c->N, Nh = c->Nh;
+      const float invN3 = (float)(1.0 / ((double)N * N * N));
+      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
+                               c->d_tw_full, c->stream);
+      if (e == hipSuccess) e = launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, c->stream);
+      if (e == hipSuccess) e = launch_inv_z(Nh, c->d_T, (long long)N * N, c->d_tw_half, c->d_tw_full, c->stream);
+      if (e != hipSuccess) rc = c->hip_fail(e, "smoothed field");
+    }
+  }
+  c->timing = was_timing;
+  if (rc) return rc;
+  return copy_field_out(c, reinterpret_cast<const float*>(c->d_T), out, PRC_D2Z_DENSE);
+}
+
+int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  c->timing = enable != 0;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_timing(prc_d2z_ctx* c, prc_d2z_timing* o) {
+  if (!c || !o) return PRC_D2Z_ERR_BAD_ARG;
+  memset(o, 0, sizeof(*o));
+  if (!c->timer.t0 || c->timer.recs.empty()) return c->fail(PRC_D2Z_ERR_STATE, "no timed run");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "timing");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->timer.t0, c->timer.t1);
+  o->total_ms = ms;
+  const int ncat = std::min((int)c->timer.cats.size(), PRC_D2Z_MAX_KERNELS);
+  o->nkernels = ncat;
+  for (int k = 0; k < ncat; ++k) strncpy(o->name[k], c->timer.cats[k].c_str(), sizeof(o->name[k]) - 1);
+  for (auto& rec : c->timer.recs) {
+    if (rec.cat >= ncat) continue;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, rec.a, rec.b);
+    o->ms[rec.cat] += t;
+    o->launches[rec.cat] += 1;
+  }
+  return PRC_D2Z_OK;
+}
+
+}  // extern "C"

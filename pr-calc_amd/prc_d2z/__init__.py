@@ -1,0 +1,235 @@
+"""Python host binding of libprc_d2z (ctypes over the C ABI in include/prc_d2z.h).
+
+Mirrors the reference d2z program's operator interface for the hot path
+(marcelo-alvarez/pr-calc, c++/src/main_d2z.C:15-109):
+
+    params = read_parameter_file("param.d2z")      # io_d2z.C:16-162
+    apply_command_line(params, ["-z", "40", ...])   # commandline_d2z.C:33-99
+    with D2Z(params) as d:                          # FFTW sizing/alloc/plans
+        d.set_delta(read_delta_file(...))           # ReadDeltaFile
+        d.run()                                     # forward r2c + delta2zreion()
+        z = d.zreion()                              # WriteZreionFile's source
+
+The library is required: importing works without it, but constructing a D2Z
+raises if ``libprc_d2z.so`` is missing or no GPU is present.  There is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .params import (D2ZParams, apply_command_line, read_parameter_file,  # noqa: F401
+                     read_delta_file, write_zreion_file)
+
+LIB_PATH = Path(__file__).resolve().parents[1] / "lib" / "libprc_d2z.so"
+
+PLAIN_FP32, FUSED_FP32, FUSED_MIXED = 0, 1, 2
+DENSE, PADDED = 0, 1
+NHISTORY = 100000
+MAX_KERNELS = 16
+
+STATUS = {
+    0: "OK", -1: "BAD_ARG", -2: "UNSUPPORTED_N", -3: "OOM", -4: "HIP",
+    -5: "NAN_INPUT", -6: "STATE", -7: "NO_DEVICE",
+}
+
+
+class CParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("Omegam", "Omegab", "Omegal", "h", "ns", "sigma8", "w",
+                                         "BoxSize", "zInit", "Rmin")] + \
+               [("N", C.c_int), ("Nscales", C.c_int), ("RecordBubbleSizes", C.c_int)] + \
+               [(n, C.c_float) for n in ("Rmax", "zeta", "M_min")] + \
+               [("mode", C.c_int), ("radii_per_launch", C.c_int), ("device", C.c_int)]
+
+
+class CScalars(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("DInit", "DHigh", "A", "pnorm", "erfizeta", "sigma_min")]
+
+
+class CRadiusRow(C.Structure):
+    _fields_ = [("R_mpch", C.c_float), ("sigma", C.c_float), ("sigma_grid", C.c_float)]
+
+
+class CTiming(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("nkernels", C.c_int),
+                ("name", (C.c_char * 32) * MAX_KERNELS),
+                ("ms", C.c_double * MAX_KERNELS), ("launches", C.c_int * MAX_KERNELS)]
+
+
+EXPORTS = [
+    "prc_d2z_abi_version", "prc_d2z_default_params", "prc_d2z_create", "prc_d2z_destroy",
+    "prc_d2z_last_error", "prc_d2z_set_delta", "prc_d2z_set_delta_device", "prc_d2z_run",
+    "prc_d2z_run_async", "prc_d2z_sync", "prc_d2z_get_zreion", "prc_d2z_get_rbubble",
+    "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
+    "prc_d2z_smoothed_field", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
+]
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None):
+    """Load libprc_d2z.so (raises OSError when it is missing)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise OSError(f"libprc_d2z.so not built at {p}: run __graft_entry__.build()")
+    lib = C.CDLL(str(p))
+    vp = C.c_void_p
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    sig = {
+        "prc_d2z_abi_version": (C.c_int, []),
+        "prc_d2z_default_params": (None, [C.POINTER(CParams)]),
+        "prc_d2z_create": (C.c_int, [C.POINTER(CParams), C.POINTER(vp)]),
+        "prc_d2z_destroy": (None, [vp]),
+        "prc_d2z_last_error": (C.c_char_p, [vp]),
+        "prc_d2z_set_delta": (C.c_int, [vp, fp, C.c_int]),
+        "prc_d2z_set_delta_device": (C.c_int, [vp, vp, C.c_int]),
+        "prc_d2z_run": (C.c_int, [vp]),
+        "prc_d2z_run_async": (C.c_int, [vp]),
+        "prc_d2z_sync": (C.c_int, [vp]),
+        "prc_d2z_get_zreion": (C.c_int, [vp, fp, C.c_int]),
+        "prc_d2z_get_rbubble": (C.c_int, [vp, fp, C.c_int]),
+        "prc_d2z_get_scalars": (C.c_int, [vp, C.POINTER(CScalars)]),
+        "prc_d2z_get_radius_table": (C.c_int, [vp, C.POINTER(CRadiusRow), C.c_int]),
+        "prc_d2z_zreion_device": (C.c_int, [vp, C.POINTER(vp)]),
+        "prc_d2z_history": (C.c_int, [vp, up]),
+        "prc_d2z_smoothed_field": (C.c_int, [vp, C.c_int, fp]),
+        "prc_d2z_set_timing": (C.c_int, [vp, C.c_int]),
+        "prc_d2z_get_timing": (C.c_int, [vp, C.POINTER(CTiming)]),
+        "prc_d2z_device_bytes": (C.c_size_t, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class D2ZError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"prc_d2z {STATUS.get(code, code)} ({code}): {msg}")
+        self.code = code
+
+
+def _to_cparams(p: D2ZParams) -> CParams:
+    c = CParams()
+    for name, _ in CParams._fields_:
+        setattr(c, name, getattr(p, name))
+    return c
+
+
+class D2Z:
+    """One box on one GPU: the reference's delta2zreion operator."""
+
+    def __init__(self, params: D2ZParams):
+        self.lib = load_library()
+        self.params = params
+        self._ctx = C.c_void_p()
+        cp = _to_cparams(params)
+        rc = self.lib.prc_d2z_create(C.byref(cp), C.byref(self._ctx))
+        if rc:
+            raise D2ZError(rc, self.lib.prc_d2z_last_error(None).decode())
+        self.N = params.N
+
+    # ---------------------------------------------------------------- utils
+    def _check(self, rc: int):
+        if rc:
+            raise D2ZError(rc, self.lib.prc_d2z_last_error(self._ctx).decode())
+
+    def close(self):
+        if self._ctx:
+            self.lib.prc_d2z_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ interface
+    def set_delta(self, delta: np.ndarray):
+        N = self.N
+        a = np.ascontiguousarray(delta, dtype=np.float32)
+        if a.shape == (N, N, N):
+            layout = DENSE
+        elif a.shape == (N, N, N + 2):
+            layout = PADDED
+        elif a.size == N ** 3:
+            a, layout = a.reshape(N, N, N), DENSE
+        else:
+            raise ValueError(f"delta must hold {N}^3 cells (dense) or {N}x{N}x{N + 2} (padded)")
+        self._check(self.lib.prc_d2z_set_delta(self._ctx, a, layout))
+
+    def set_delta_device(self, ptr: int, layout: int = DENSE):
+        self._check(self.lib.prc_d2z_set_delta_device(self._ctx, C.c_void_p(ptr), layout))
+
+    def run(self):
+        self._check(self.lib.prc_d2z_run(self._ctx))
+
+    def run_async(self):
+        self._check(self.lib.prc_d2z_run_async(self._ctx))
+
+    def sync(self):
+        self._check(self.lib.prc_d2z_sync(self._ctx))
+
+    def zreion(self, layout: int = DENSE) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N + 2) if layout == PADDED else (N, N, N), dtype=np.float32)
+        self._check(self.lib.prc_d2z_get_zreion(self._ctx, out, layout))
+        return out
+
+    def rbubble(self, layout: int = DENSE) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N + 2) if layout == PADDED else (N, N, N), dtype=np.float32)
+        self._check(self.lib.prc_d2z_get_rbubble(self._ctx, out, layout))
+        return out
+
+    def scalars(self) -> dict:
+        s = CScalars()
+        self._check(self.lib.prc_d2z_get_scalars(self._ctx, C.byref(s)))
+        return {n: np.float32(getattr(s, n)) for n, _ in CScalars._fields_}
+
+    def radius_table(self) -> np.ndarray:
+        n = self.params.Nscales
+        rows = (CRadiusRow * n)()
+        self._check(self.lib.prc_d2z_get_radius_table(self._ctx, rows, n))
+        return np.array([(r.R_mpch, r.sigma, r.sigma_grid) for r in rows], dtype=np.float32)
+
+    def history(self) -> np.ndarray:
+        out = np.zeros(NHISTORY, dtype=np.uint32)
+        self._check(self.lib.prc_d2z_history(self._ctx, out))
+        return out
+
+    def smoothed_field(self, r: int) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N), dtype=np.float32)
+        self._check(self.lib.prc_d2z_smoothed_field(self._ctx, r, out))
+        return out
+
+    def set_timing(self, on: bool = True):
+        self._check(self.lib.prc_d2z_set_timing(self._ctx, int(on)))
+
+    def timing(self) -> dict:
+        t = CTiming()
+        self._check(self.lib.prc_d2z_get_timing(self._ctx, C.byref(t)))
+        ker = {t.name[k].value.decode(): (t.ms[k], t.launches[k]) for k in range(t.nkernels)}
+        return {"total_ms": t.total_ms, "kernels": ker}
+
+    def device_bytes(self) -> int:
+        return int(self.lib.prc_d2z_device_bytes(self._ctx))
