@@ -288,8 +288,10 @@ def radius_schedule(p: D2ZParams):
     box = f32(f32(p.BoxSize) / f32(p.h))
     Rinitial = f32(f32(box * f32(p.Rmin)) / f32(p.N))
     Rfinal = f32(f32(p.Rmax) / f32(p.h))
-    dScale = f32(f32(np.log10(Rfinal) - np.log10(Rinitial)) / f32(p.Nscales - 1))
-    l0 = f32(np.log10(Rinitial))
+    # log10 of a float, correctly rounded (what glibc's log10f returns)
+    l0 = f32(math.log10(f64(Rinitial)))
+    l1 = f32(math.log10(f64(Rfinal)))
+    dScale = f32(f32(l1 - l0) / f32(p.Nscales - 1))
     out = np.empty(p.Nscales, dtype=np.float32)
     for r in range(p.Nscales):
         out[r] = f32(10.0 ** f64(f32(l0 + f32(f32(r) * dScale))))

@@ -376,8 +376,10 @@ __device__ float radius_of(const SetupParams& p, int r) {
   const float box = p.BoxSize / p.h;
   const float Rinitial = (box * p.Rmin) / (float)p.N;
   const float Rfinal = p.Rmax / p.h;
-  const float dScale = (log10f(Rfinal) - log10f(Rinitial)) / (float)(p.Nscales - 1);
-  const float e = log10f(Rinitial) + (float)r * dScale;
+  // log10 of a float, correctly rounded (double evaluation then float)
+  const float l0 = (float)log10((double)Rinitial), l1 = (float)log10((double)Rfinal);
+  const float dScale = (l1 - l0) / (float)(p.Nscales - 1);
+  const float e = l0 + (float)r * dScale;
   return (float)pow(10.0, (double)e);
 }
 
