@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the delta2zreion hot path on one MI355X.
+
+One step = one full delta2zreion pass over a box resident in HBM: forward r2c
+of the density field, then for every radius the filter, c2r, barrier and
+running maximum (the work of c++/src/main_d2z.C:57-71 in
+marcelo-alvarez/pr-calc).  Default workload: 1024^3 cells, 2000 Mpc/h box,
+96 radii from one cell to 120 Mpc/h, zeta 40, M_min 1e8 Msun/h, input a
+synthetic Gaussian random field generated on the GPU (seed 1).
+
+Prints ONE JSON line.  Extra objects:
+  roofline      dominant kernel: algorithmic bytes per launch / average launch
+                time (HIP events on the library's stream during the timed
+                steps) against the 8 TB/s HBM peak
+  cpu_baseline  the CPU restatement timed on a bounded sample of the same
+                workload on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "pr-calc_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+B_ALG_PATH = 8.0  # algorithmic bytes per cell-radius of the whole path
+
+WORKLOADS = {
+    # name: N, BoxSize [Mpc/h], Nscales, Rmax [Mpc/h]
+    "c1_128": (128, 200.0, 32, 50.0),
+    "c2_512": (512, 1000.0, 64, 80.0),
+    "c3_1024": (1024, 2000.0, 96, 120.0),
+}
+MODES = {"plain": 0, "fused": 1, "mixed": 2}
+
+
+def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:
+    """Algorithmic HBM bytes one launch of a kernel must move.
+
+    Intermediates hold N^3/2 complex = N^3 * store_bytes/... per radius:
+    fx_filter   reads the half spectrum once (4 B/cell) and writes nb
+                intermediates (store B/cell each)
+    inv_y       reads and writes nb intermediates
+    fz_barrier  reads nb intermediates and reads+writes zreion (8 B/cell)
+    plain kernels: filter 8, inv_x/inv_y/inv_z_c2r 8, barrier 12, moments 4 B/cell
+    """
+    cells = float(N) ** 3
+    s = float(store_bytes)
+    table = {
+        "fx_filter": 4.0 + s * nb,
+        "inv_y": 2 * s * nb,
+        "fz_barrier": s * nb + 8.0,
+        "filter": 8.0, "inv_x": 8.0, "inv_z_c2r": 8.0, "barrier": 12.0, "moments": 4.0,
+        "fwd_z_r2c": 8.0, "fwd_y": 8.0, "fwd_x": 8.0,
+    }
+    return table.get(name, 0.0) * cells
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    f = ROOT / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline_sample(delta: np.ndarray, lp, nthreads: int) -> dict:
+    """Time the oracle-side CPU restatement on a bounded sample: the forward
+    transform plus one radius of the same box; the full-run rate is
+    extrapolated as N^3 Nscales / (t_forward + Nscales t_radius)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import d2z_cpu  # oracle-side C restatement (float, OpenMP)
+
+    N, NS = lp.N, lp.Nscales
+    res = d2z_cpu.time_sample(delta, lp, radii=[NS // 2], nthreads=nthreads)
+    t_fwd, t_rad = res["t_forward"], res["t_radius"]
+    rate = N ** 3 * NS / (t_fwd + NS * t_rad)
+    return {"value": rate, "unit": "cell*radius/s", "cores": nthreads, "kind": "port",
+            "sample": f"{N}^3 forward r2c + 1 of {NS} radii (r={NS // 2}) with the C FP32 restatement "
+                      f"({res['fft']} FFT); t_forward={t_fwd:.2f}s t_radius={t_rad:.2f}s, "
+                      f"rate extrapolated to {NS} radii"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="fused", choices=list(MODES))
+    ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS))
+    ap.add_argument("--radii-per-launch", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: the delta2zreion path runs the whole grid on a single "
+              f"MI355X; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from prc_d2z import D2Z, D2ZParams
+
+    N, box, NS, rmax = WORKLOADS[args.workload]
+    lp = D2ZParams(N=N, BoxSize=box, Nscales=NS, Rmax=rmax, zeta=40.0, M_min=1.4706e8,
+                   mode=MODES[args.mode], radii_per_launch=args.radii_per_launch)
+    d = D2Z(lp)
+    d.generate_delta(seed=1)          # input resident in HBM before timing
+    for _ in range(args.warmup):
+        d.run()
+    d.set_timing(True)                # HIP events around every launch on the library's stream
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d.run_async()
+        kt = None
+    d.sync()
+    t1 = time.perf_counter()
+    kt = d.timing()                   # events of the last timed step
+    d.set_timing(False)
+    elapsed = t1 - t0
+    ms_step = elapsed / args.steps * 1e3
+    value = N ** 3 * NS * args.steps / elapsed
+
+    kernels = kt["kernels"]
+    dom = max(kernels.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_n) = dom
+    nb = min(8, NS) if args.mode != "plain" else 1
+    if args.radii_per_launch:
+        nb = args.radii_per_launch
+    store = 2 if args.mode == "mixed" else 4
+    alg = kernel_alg_bytes(dom_name, N, nb, store)
+    avg_s = dom_ms / dom_n * 1e-3
+    achieved = alg / avg_s / 1e9 if alg else None
+    roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": traffic_from_profiles(dom_name, args.workload),
+            "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": alg,
+            "path_achieved": value * B_ALG_PATH / 1e9,
+            "path_frac": value * B_ALG_PATH / 1e9 / HBM_PEAK_GBS}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            delta = d.delta()
+            cpu = cpu_baseline_sample(delta, lp, args.cpu_threads)
+        except Exception as e:  # the baseline is reported, not required
+            cpu = {"value": None, "unit": "cell*radius/s", "cores": args.cpu_threads, "kind": "port",
+                   "sample": f"unavailable: {e}"}
+    d.close()
+
+    out = {
+        "metric": "cell*radius updates/s at 1024^3 x 96 radii (delta2zreion)",
+        "value": value, "unit": "cell*radius/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.mode != "mixed" else "f32/f16",
+        "data": "synthetic",
+        "config": {"workload": f"{N}^3 box {box:g} Mpc/h, {NS} radii 1 cell..{rmax:g} Mpc/h, "
+                               f"zeta 40, Mmin 1e8 Msun/h, GRF seed 1", "mode": args.mode,
+                   "radii_per_launch": nb},
+        "roofline": roof,
+        "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,16 @@ int prc_d2z_set_delta(prc_d2z_ctx* ctx, const float* delta, int layout);
 /* Same from device memory already on the context's device. */
 int prc_d2z_set_delta_device(prc_d2z_ctx* ctx, const float* d_delta, int layout);
 
+/* Synthetic linear density field generated on the device into the context's
+ * input buffer (replaces the ics program's output, c++/src/grf_ics.C:8-112):
+ * Philox4x32-10 normals keyed by (seed, cell), coloured by
+ * sqrt(P(k)/V_cell) with P = A k^ns T_BBKS(k)^2 (Sugiyama shape) normalised to
+ * sigma8 from the context's cosmology and BoxSize.  white_noise_only = 1 stops
+ * after the normals. */
+int prc_d2z_generate_delta(prc_d2z_ctx* ctx, uint64_t seed, int white_noise_only);
+/* Copy the context's current real-space input field back to the host. */
+int prc_d2z_get_delta(prc_d2z_ctx* ctx, float* out, int layout);
+
 /* main_d2z.C:57-71: delta /= DInit, forward r2c, then delta2zreion() for all
  * radii.  zreion starts from 0 (allocate_d2z.C:37) on every call. */
 int prc_d2z_run(prc_d2z_ctx* ctx);

@@ -27,6 +27,14 @@ struct RadiusConst {
   double B;     // sqrt(2 (sigma_min^2 - sigma^2)) * erfizeta, double
 };
 
+// Synthetic field spectrum: BBKS with Sugiyama shape, k in h/Mpc.
+struct GrfSpectrum {
+  double gamma, ns, sigma8;
+  double dk;     // 2 pi / BoxSize [h/Mpc]
+  double vcell;  // (BoxSize/N)^3 [(Mpc/h)^3]
+  double invN3;  // 1/N^3 of the inverse transform
+};
+
 hipError_t launch_twiddles(float2* tw, int L, hipStream_t s);
 hipError_t launch_fwd_z(int Nh, const float* in, long long in_row_stride, float2* out, long long nrows,
                         float dinit, const float2* tw_half, const float2* tw_full, hipStream_t s);
@@ -57,5 +65,11 @@ hipError_t launch_count_nonfinite(const float* x, long long n, int* flag, hipStr
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
 hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
                                hipStream_t s);
+
+// synthetic input field
+hipError_t launch_white_noise(float* out, long long n, unsigned long long seed, hipStream_t s);
+hipError_t launch_grf_norm(const GrfSpectrum& sp, double* A, hipStream_t s);
+hipError_t launch_colour(float2* S, const float2* D0, const float2* DN, int N, const GrfSpectrum& sp,
+                         const double* A, hipStream_t s);
 
 }  // namespace prc

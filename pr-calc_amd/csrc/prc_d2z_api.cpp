@@ -100,6 +100,7 @@ struct prc_d2z_ctx {
   RadiusConst* d_rc = nullptr;
   SetupScalars* d_sc = nullptr;
   double* d_integ = nullptr;
+  double* d_grfA = nullptr;
   double* d_Rint = nullptr;
   float* d_R = nullptr;
   unsigned* d_hist = nullptr;
@@ -156,11 +157,11 @@ hipError_t timed(prc_d2z_ctx* c, const char* name, F&& f) {
     if (e_ != hipSuccess) return (ctx)->hip_fail(e_, what); \
   } while (0)
 
-int forward(prc_d2z_ctx* c) {
+int forward(prc_d2z_ctx* c, float dinit) {
   const int N = c->N, Nh = c->Nh;
   hipStream_t s = c->stream;
   PRC_CHECK(c, timed(c, "fwd_z_r2c", [&] {
-              return launch_fwd_z(Nh, c->d_delta, N, c->d_S, (long long)N * N, c->sc.DInit, c->d_tw_half,
+              return launch_fwd_z(Nh, c->d_delta, N, c->d_S, (long long)N * N, dinit, c->d_tw_half,
                                   c->d_tw_full, s);
             }),
             "forward z pass");
@@ -251,7 +252,7 @@ int run_all(prc_d2z_ctx* c) {
     }
     (void)hipEventRecord(c->timer.t0, s);
   }
-  int rc = forward(c);
+  int rc = forward(c, c->sc.DInit);
   if (rc) return rc;
   PRC_CHECK(c, launch_fill(c->d_zre, c->ncell, 0.0f, s), "zreion init");
   if (c->p.RecordBubbleSizes) PRC_CHECK(c, launch_fill(c->d_rb, c->ncell, 0.0f, s), "rbubble init");
@@ -431,6 +432,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_R, sizeof(float) * NS);
   A_(c->d_hist, sizeof(unsigned) * kNHistory);
   A_(c->d_flag, sizeof(int));
+  A_(c->d_grfA, sizeof(double));
 #undef A_
   if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("device allocation failed: ") + hipGetErrorString(e));
   // twiddle tables and the FP64 scalar set-up, all on the device
@@ -481,6 +483,50 @@ int prc_d2z_set_delta(prc_d2z_ctx* c, const float* delta, int layout) {
 
 int prc_d2z_set_delta_device(prc_d2z_ctx* c, const float* d_delta, int layout) {
   return set_delta_impl(c, d_delta, layout, hipMemcpyDeviceToDevice);
+}
+
+int prc_d2z_generate_delta(prc_d2z_ctx* c, uint64_t seed, int white_noise_only) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  (void)hipSetDevice(c->p.device);
+  const int N = c->N, Nh = c->Nh;
+  hipStream_t s = c->stream;
+  const bool was_timing = c->timing;
+  c->timing = false;
+  c->have_delta = c->have_spectrum = c->have_result = false;
+  PRC_CHECK(c, launch_white_noise(c->d_delta, c->ncell, (unsigned long long)seed, s), "white noise");
+  if (!white_noise_only) {
+    const double om = c->p.Omegam, ob = c->p.Omegab, h = c->p.h, box = c->p.BoxSize;
+    GrfSpectrum sp;
+    sp.gamma = om * h * exp(-ob * (1.0 + sqrt(2.0 * h) / om));
+    sp.ns = c->p.ns;
+    sp.sigma8 = c->p.sigma8;
+    sp.dk = 2.0 * M_PI / box;
+    sp.vcell = pow(box / N, 3.0);
+    sp.invN3 = 1.0 / ((double)N * N * N);
+    int rc = forward(c, 1.0f);
+    if (rc) {
+      c->timing = was_timing;
+      return rc;
+    }
+    PRC_CHECK(c, launch_grf_norm(sp, c->d_grfA, s), "field normalisation");
+    PRC_CHECK(c, launch_colour(c->d_S, c->d_D0, c->d_DN, N, sp, c->d_grfA, s), "colouring");
+    PRC_CHECK(c, launch_strided(N, +1, c->d_S, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s), "field x pass");
+    PRC_CHECK(c, launch_strided(N, +1, c->d_S, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s), "field y pass");
+    PRC_CHECK(c, launch_inv_z(Nh, c->d_S, (long long)N * N, c->d_tw_half, c->d_tw_full, s), "field z pass");
+    PRC_CHECK(c, hipMemcpyAsync(c->d_delta, c->d_S, sizeof(float) * c->ncell, hipMemcpyDeviceToDevice, s),
+              "field copy");
+  }
+  PRC_CHECK(c, hipStreamSynchronize(s), "field generation");
+  c->timing = was_timing;
+  c->have_spectrum = false;
+  c->have_delta = true;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_delta(prc_d2z_ctx* c, float* out, int layout) {
+  if (!c || !out) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "null output") : PRC_D2Z_ERR_BAD_ARG;
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field");
+  return copy_field_out(c, c->d_delta, out, layout);
 }
 
 int prc_d2z_run_async(prc_d2z_ctx* c) {
@@ -569,7 +615,7 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
   (void)hipSetDevice(c->p.device);
   const bool was_timing = c->timing;
   c->timing = false;
-  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c);
+  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c, c->sc.DInit);
   if (!rc) {
     if (c->p.mode == PRC_D2Z_PLAIN_FP32) {
       rc = plain_radius(c, r, false);

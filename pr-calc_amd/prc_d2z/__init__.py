@@ -66,6 +66,7 @@ EXPORTS = [
     "prc_d2z_run_async", "prc_d2z_sync", "prc_d2z_get_zreion", "prc_d2z_get_rbubble",
     "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
     "prc_d2z_smoothed_field", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
+    "prc_d2z_generate_delta", "prc_d2z_get_delta",
 ]
 
 _lib = None
@@ -104,6 +105,8 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_set_timing": (C.c_int, [vp, C.c_int]),
         "prc_d2z_get_timing": (C.c_int, [vp, C.POINTER(CTiming)]),
         "prc_d2z_device_bytes": (C.c_size_t, [vp]),
+        "prc_d2z_generate_delta": (C.c_int, [vp, C.c_uint64, C.c_int]),
+        "prc_d2z_get_delta": (C.c_int, [vp, fp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -178,6 +181,16 @@ class D2Z:
 
     def set_delta_device(self, ptr: int, layout: int = DENSE):
         self._check(self.lib.prc_d2z_set_delta_device(self._ctx, C.c_void_p(ptr), layout))
+
+    def generate_delta(self, seed: int, white_noise_only: bool = False):
+        """Synthetic GRF on the device (Philox normals, BBKS spectrum)."""
+        self._check(self.lib.prc_d2z_generate_delta(self._ctx, seed, int(white_noise_only)))
+
+    def delta(self, layout: int = DENSE) -> np.ndarray:
+        N = self.N
+        out = np.empty((N, N, N + 2) if layout == PADDED else (N, N, N), dtype=np.float32)
+        self._check(self.lib.prc_d2z_get_delta(self._ctx, out, layout))
+        return out
 
     def run(self):
         self._check(self.lib.prc_d2z_run(self._ctx))
