@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--radii-per-launch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--history", action="store_true",
+                    help="also run the GetHistory histogram once after timing (read-only pass over zreion)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -124,7 +126,6 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         d.run_async()
-        kt = None
     d.sync()
     t1 = time.perf_counter()
     kt = d.timing()                   # events of the last timed step
@@ -150,6 +151,9 @@ def main():
             "path_achieved": value * B_ALG_PATH / 1e9,
             "path_frac": value * B_ALG_PATH / 1e9 / HBM_PEAK_GBS}
 
+    if args.history:
+        h = d.history()
+        print(f"history: {int(h.sum())} cells counted", file=sys.stderr)
     cpu = None
     if not args.no_cpu_baseline:
         try:

@@ -328,14 +328,15 @@ def setup_scalars(p: D2ZParams) -> Scalars:
 # --------------------------------------------------------------------------
 def tophat_W(x):
     """W(x) = 3 (sin x - x cos x)/x^3, evaluated with its Taylor series
-    1 - x^2/10 + x^4/280 - x^6/15120 + x^8/1330560 for x < 0.5 (the reference
-    evaluates the closed form in float, delta2zreion_d2z.C:76-77)."""
+    1 - x^2/10 + x^4/280 - x^6/15120 + ... (term ratio -x^2/(2n(2n+3)), through
+    x^12) for x < 0.5 (the reference evaluates the closed form in float,
+    delta2zreion_d2z.C:76-77)."""
     x = np.asarray(x, dtype=np.float64)
     out = np.empty_like(x)
     small = x < 0.5
     xs = x[small]
     x2 = xs * xs
-    out[small] = 1 - x2 / 10 * (1 - x2 / 28 * (1 - x2 / 54 * (1 - x2 / 88)))
+    out[small] = 1 - x2 / 10 * (1 - x2 / 28 * (1 - x2 / 54 * (1 - x2 / 88 * (1 - x2 / 130 * (1 - x2 / 180)))))
     xl = x[~small]
     out[~small] = 3 * (np.sin(xl) - xl * np.cos(xl)) / xl ** 3
     return out
@@ -363,7 +364,7 @@ def smooth(dhat, kmag, R):
     N = dhat.shape[0]
     W = tophat_W(kmag * f64(R))
     W.flat[0] = 0.0
-    return np.fft.irfftn(dhat * W, s=(N, N, N))
+    return np.fft.irfftn(dhat * W, s=(N, N, N), axes=(0, 1, 2))
 
 
 def sigma_grid(dR):

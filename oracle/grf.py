@@ -124,4 +124,4 @@ def grf(N, box_mpch, seed, omegam=0.31, omegab=0.049, h=0.68, ns=0.965, sigma8=0
     with np.errstate(divide="ignore", invalid="ignore"):
         amp = np.sqrt(power(k, omegam, omegab, h, ns, sigma8) / vcell)
     amp.flat[0] = 0.0
-    return np.fft.irfftn(nk * amp, s=(N, N, N)).astype(np.float32)
+    return np.fft.irfftn(nk * amp, s=(N, N, N), axes=(0, 1, 2)).astype(np.float32)

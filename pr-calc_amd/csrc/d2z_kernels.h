@@ -35,16 +35,33 @@ struct GrfSpectrum {
   double invN3;  // 1/N^3 of the inverse transform
 };
 
+constexpr int kFusedMaxRadii = 8;  // radii per fused batch
+
+// Strides of one tiled pass (complex elements); see pass_kernels.hip.
+struct TileMap {
+  long long in_sA, in_sO, out_sA, out_sO, sCB;
+  int nOuter, nColBlocks;
+};
+enum class Axis { kYInPlace, kXTtoS, kXStoT };
+int block_width(int N);
+TileMap tile_map(int N, Axis axis);
+
 hipError_t launch_twiddles(float2* tw, int L, hipStream_t s);
-hipError_t launch_fwd_z(int Nh, const float* in, long long in_row_stride, float2* out, long long nrows,
-                        float dinit, const float2* tw_half, const float2* tw_full, hipStream_t s);
-hipError_t launch_inv_z(int Nh, float2* data, long long nrows, const float2* tw_half, const float2* tw_full,
+// pencil passes (pass_kernels.hip)
+hipError_t launch_fwd_z(int N, const float* in, float2* out, float dinit, const float2* twh, const float2* twf,
                         hipStream_t s);
-hipError_t launch_strided(int L, int dir, float2* data, long long sA, long long sO, long long nOuter, int ncols,
-                          const float2* tw, hipStream_t s);
-hipError_t launch_unpack(const float2* S, float2* D0, float2* DN, int N, hipStream_t s);
-hipError_t launch_filter(const float2* S, const float2* D0, const float2* DN, float2* out, int N, float dkR,
+hipError_t launch_inv_z(int N, const float2* in, float* out, const float2* twh, const float2* twf, hipStream_t s);
+hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const TileMap& mp, const float2* tw,
+                       hipStream_t s);
+hipError_t launch_unpack(int N, const float2* S, float2* D0, float2* DN, hipStream_t s);
+hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
                          float invN3, hipStream_t s);
+hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
+                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
+int fz_blocks(int N);
+hipError_t launch_fz(int N, const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                     double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s);
+// element-wise and set-up kernels (d2z_kernels.hip)
 hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long total, const RadiusConst* rc, int r,
                           hipStream_t s);
 hipError_t launch_moments(const float* x, long long total, double2* partial, int nblk, hipStream_t s);
@@ -54,13 +71,6 @@ hipError_t launch_history(const float* zre, long long total, unsigned int* count
 hipError_t launch_setup(const SetupParams& p, double* integ, float* Rr, double* Rint, SetupScalars* sc,
                         RadiusConst* rc, float* sigma_out, hipStream_t s);
 
-// fused path
-hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
-                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
-int fz_blocks(int Nh, long long nrows);
-hipError_t launch_fz(int Nh, const float2* Tbuf, float* zre, float* rbub, long long nrows, const RadiusConst* rc,
-                     int r0, int nb, double* partial, int nblk, const float2* tw_half, const float2* tw_full,
-                     hipStream_t s);
 hipError_t launch_count_nonfinite(const float* x, long long n, int* flag, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
 hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
@@ -69,7 +79,7 @@ hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double
 // synthetic input field
 hipError_t launch_white_noise(float* out, long long n, unsigned long long seed, hipStream_t s);
 hipError_t launch_grf_norm(const GrfSpectrum& sp, double* A, hipStream_t s);
-hipError_t launch_colour(float2* S, const float2* D0, const float2* DN, int N, const GrfSpectrum& sp,
+hipError_t launch_colour(int N, float2* S, const float2* D0, const float2* DN, const GrfSpectrum& sp,
                          const double* A, hipStream_t s);
 
 }  // namespace prc

@@ -1,22 +1,17 @@
-// HIP kernels for the delta2zreion path on gfx950.
+// Set-up and element-wise kernels of the delta2zreion path (gfx950): twiddle
+// tables, the FP64 scalar set-up (sigma(R) quadrature, erfc^-1 root, growth
+// factors), the plain path's barrier and moments, the history histogram.
 //
-// Data layout in HBM (all row-major, last index fastest):
-//   real fields        float  [N][N][N]           (x, y, z)
-//   half spectrum S    float2 [N][N][N/2]         (kx, ky, kz) with the kz = 0
-//       slot holding the packed pair  H(kz=0) + i*H(kz=N/2)  of the z-transform
-//       (both are real along each z row, so one complex slot carries them);
-//   side planes D0, DN float2 [N][N]              the true spectrum at kz = 0
-//       and kz = N/2, unpacked once after the forward transform.
-//
-// Reference: c++/src/main_d2z.C:57-67 (scale, forward r2c) and
-// c++/src/delta2zreion_d2z.C:8-216 (per-radius filter, c2r, normalise,
-// sigma_grid, barrier, running max).
+// Reference: c++/src/delta2zreion_d2z.C:12-38,97-146,206-216,
+// c++/src/cosmology.C:174-309, c++/src/math.C:23-61,
+// c++/src/arrayoperations.C:4-35, c++/src/bubblehistory_d2z.C:5-41.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
-#include "fft_core.h"
+#include "d2z_device.h"
 #include "d2z_kernels.h"
+#include "fft_core.h"
 
 namespace prc {
 
@@ -29,223 +24,6 @@ __global__ void k_twiddles(float2* __restrict__ tw, int L) {
     sincospi(2.0 * (double)m / (double)L, &s, &c);
     tw[m] = make_float2((float)c, (float)s);
   }
-}
-
-// ---------------------------------------------------- top-hat filter W(kR)
-// W(x) = 3 (sin x - x cos x) / x^3; Taylor series below x = 1 where the closed
-// form cancels (terms to x^10: truncation < 4e-11 on [0,1)).
-__device__ __forceinline__ float tophat(float x) {
-  if (x < 1.0f) {
-    float x2 = x * x;
-    return 1.0f - x2 * (1.0f / 10.0f -
-                        x2 * (1.0f / 280.0f -
-                              x2 * (1.0f / 15120.0f - x2 * (1.0f / 1330560.0f - x2 * (1.0f / 172972800.0f)))));
-  }
-  float s, c;
-  sincosf(x, &s, &c);
-  return 3.0f * (s - x * c) / (x * x * x);
-}
-
-// signed frequency index of grid index i (i >= N/2 maps to i - N), squared
-__device__ __forceinline__ int fsq(int i, int N) {
-  int f = (i >= (N >> 1)) ? i - N : i;
-  return f * f;
-}
-
-// W(|k| R) / N^3 for integer |k|^2 = m dk^2; k = 0 gives 0 (the reference
-// zeroes the mean mode, delta2zreion_d2z.C:79-85).
-__device__ __forceinline__ float filter_coef(int m, float dkR, float invN3) {
-  if (m == 0) return 0.0f;
-  return tophat(sqrtf((float)m) * dkR) * invN3;
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
-// Remap so each XCD works on a contiguous range of tiles: neighbouring
-// tiles that share 128 B lines then meet in the same L2.
-__device__ __forceinline__ long long xcd_tile(long long b, long long nblk) {
-  if (nblk % 8 != 0) return b;
-  return (b % 8) * (nblk / 8) + b / 8;
-}
-
-// ------------------------------------------------ forward z pass (r2c)
-// Row of N reals -> packed half spectrum of N/2 complex.  The row is read as
-// N/2 complex z[n] = x[2n] + i x[2n+1] and divided by DInit
-// (main_d2z.C:57-59); after the N/2-point forward FFT Z[k]:
-//   X[k] = (Z[k] + conj Z[L-k])/2 - i exp(-2 pi i k/N) (Z[k] - conj Z[L-k])/2,
-//   slot 0 = X[0] + i X[N/2] = (Re Z0 + Im Z0) + i (Re Z0 - Im Z0).
-template <int L>
-__global__ void __launch_bounds__(256) k_fwd_z_r2c(const float* __restrict__ in, long long in_row_stride,
-                                                   float2* __restrict__ out, long long nrows, float dinit,
-                                                   const float2* __restrict__ tw_half,
-                                                   const float2* __restrict__ tw_full) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  __shared__ float2 smem[ROWS * L];
-  const int t = threadIdx.x;
-  const int i = t % T, rr = t / T;
-  const long long row = (long long)blockIdx.x * ROWS + rr;
-  const bool valid = row < nrows;
-  LdsPencil lds{smem + rr * L, 1};
-  float2 v[E];
-  const float2* src = reinterpret_cast<const float2*>(in + (valid ? row : 0) * in_row_stride);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    float2 a = valid ? src[i + e * T] : make_float2(0.f, 0.f);
-    v[e] = make_float2(a.x / dinit, a.y / dinit);
-  }
-  pencil_fft<L, -1>(v, i, tw_half, lds);
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < E; ++e) lds[i + e * T] = v[e];
-  __syncthreads();
-  float2* dst = out + (valid ? row : 0) * L;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = i + e * T;
-    float2 Z = v[e];
-    float2 X;
-    if (k == 0) {
-      X = make_float2(Z.x + Z.y, Z.x - Z.y);
-    } else {
-      float2 Zm = cconj(lds[L - k]);
-      float2 Ev = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y + Zm.y));
-      float2 D = make_float2(0.5f * (Z.x - Zm.x), 0.5f * (Z.y - Zm.y));
-      float2 Od = make_float2(D.y, -D.x);  // -i * D
-      float2 w = tw_full[k];
-      w.y = -w.y;  // exp(-2 pi i k / N)
-      X = cadd(Ev, cmul(w, Od));
-    }
-    if (valid) dst[k] = X;
-  }
-}
-
-// --------------------------------------------- strided complex pass
-// In-place 1D transform of length L along a non-contiguous axis.  Block =
-// one tile: C consecutive columns (contiguous in memory) x all L elements
-// along the axis (stride sA).  Lane -> column mapping makes every load and
-// store a run of C complex values.
-template <int L, int C, int DIR>
-__global__ void __launch_bounds__(Plan<L>::T* C) k_strided(float2* __restrict__ data, long long sA, long long sO,
-                                                            int nColBlocks, const float2* __restrict__ tw) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-  __shared__ float2 smem[L * C];
-  const int t = threadIdx.x;
-  const int c = t % C, i = t / C;
-  const long long blk = xcd_tile(blockIdx.x, gridDim.x);
-  const int cb = (int)(blk % nColBlocks);
-  const long long o = blk / nColBlocks;
-  float2* base = data + o * sO + (long long)cb * C + c;
-  LdsPencil lds{smem + c, C};
-  float2 v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = base[(long long)(i + e * T) * sA];
-  pencil_fft<L, DIR>(v, i, tw, lds);
-#pragma unroll
-  for (int e = 0; e < E; ++e) base[(long long)(i + e * T) * sA] = v[e];
-}
-
-// ---------------------------------------------------- inverse z pass (c2r)
-// Packed half-spectrum row (N/2 complex) -> N reals, in place.
-//   Z[k] = X[k] + conj X[L-k] + i exp(+2 pi i k/N) (X[k] - conj X[L-k]),
-//   Z[0] = (X0 + XN) + i (X0 - XN),  z = IFFT_L(Z),  x[2n] + i x[2n+1] = z[n].
-template <int L>
-__device__ __forceinline__ void c2r_preprocess(float2 (&v)[Plan<L>::E], const float2* __restrict__ row, int i,
-                                               const float2* __restrict__ tw_full) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = i + e * T;
-    float2 a = v[e];
-    if (k == 0) {
-      v[e] = make_float2(a.x + a.y, a.x - a.y);
-    } else {
-      float2 b = cconj(row[L - k]);
-      float2 s = cadd(a, b), d = csub(a, b);
-      float2 w = tw_full[k];  // exp(+2 pi i k / N)
-      float2 wd = cmul(w, d);
-      v[e] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i * wd
-    }
-  }
-}
-
-template <int L>
-__global__ void __launch_bounds__(256) k_inv_z_c2r(float2* __restrict__ data, long long nrows,
-                                                   const float2* __restrict__ tw_half,
-                                                   const float2* __restrict__ tw_full) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  __shared__ float2 smem[ROWS * L];
-  const int t = threadIdx.x;
-  const int i = t % T, rr = t / T;
-  const long long row = (long long)blockIdx.x * ROWS + rr;
-  const bool valid = row < nrows;
-  float2* rp = data + (valid ? row : 0) * L;
-  LdsPencil lds{smem + rr * L, 1};
-  float2 v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = rp[i + e * T];
-  c2r_preprocess<L>(v, rp, i, tw_full);
-  pencil_fft<L, +1>(v, i, tw_half, lds);
-  __syncthreads();  // every lane of the row has read its inputs before any store
-  if (valid) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) rp[i + e * T] = v[e];
-  }
-}
-
-// ------------------------------------------------- unpack after forward
-// D0(k) = (P(k) + conj P(-k))/2,  DN(k) = -i (P(k) - conj P(-k))/2
-__global__ void k_unpack_planes(const float2* __restrict__ S, float2* __restrict__ D0, float2* __restrict__ DN,
-                                int N) {
-  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (long long)N * N) return;
-  const int kx = (int)(id / N), ky = (int)(id % N);
-  const int mx = (N - kx) % N, my = (N - ky) % N;
-  const long long Nh = N / 2;
-  float2 p = S[((long long)kx * N + ky) * Nh];
-  float2 q = cconj(S[((long long)mx * N + my) * Nh]);
-  D0[id] = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y + q.y));
-  float2 d = make_float2(0.5f * (p.x - q.x), 0.5f * (p.y - q.y));
-  DN[id] = make_float2(d.y, -d.x);
-}
-
-// ------------------------------------------------ plain path: filter
-// T[kx][ky][kz] = W(|k| R)/N^3 * S for kz >= 1; the kz = 0 slot is rebuilt
-// from the side planes as W0 D0 + i WN DN (delta2zreion_d2z.C:58-89 plus the
-// 1/N^3 of NormalizeArray, :93).
-__global__ void k_filter(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
-                         float2* __restrict__ out, int N, float dkR, float invN3) {
-  const long long Nh = N / 2;
-  const long long total = (long long)N * N * Nh;
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
-       id += (long long)gridDim.x * blockDim.x) {
-    const int kz = (int)(id % Nh);
-    const long long pl = id / Nh;
-    const int ky = (int)(pl % N), kx = (int)(pl / N);
-    const int mxy = fsq(kx, N) + fsq(ky, N);
-    float2 r;
-    if (kz != 0) {
-      r = cscale(S[id], filter_coef(mxy + kz * kz, dkR, invN3));
-    } else {
-      const float w0 = filter_coef(mxy, dkR, invN3);
-      const float wn = filter_coef(mxy + (int)(Nh * Nh), dkR, invN3);
-      const float2 a = D0[pl], b = DN[pl];
-      r = make_float2(w0 * a.x - wn * b.y, w0 * a.y + wn * b.x);
-    }
-    out[id] = r;
-  }
-}
-
-// -------------------------------------- barrier: d2z() in double, stored float
-// z = A*(delta_R + B) - 1 with A float and B = sqrt(2(smin^2 - s^2))*erfizeta in
-// double (delta2zreion_d2z.C:206-216); no contraction into an FMA, as in the
-// reference's separate multiply and subtract.
-__device__ __forceinline__ float barrier_z(float d, double A, double B) {
-  return (float)__dsub_rn(__dmul_rn(A, __dadd_rn((double)d, B)), 1.0);
 }
 
 // plain path: strict running maximum over a real field (delta2zreion_d2z.C:106-146)
@@ -266,11 +44,6 @@ __global__ void k_barrier(const float* __restrict__ dR, float* __restrict__ zre,
 
 // ------------------------------------------ sigma_grid partial sums (plain)
 // Per-block FP64 sums of delta and delta^2 (GetMeanSigma, arrayoperations.C:4-35).
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
 
 __global__ void __launch_bounds__(256) k_moments(const float* __restrict__ x, long long total,
                                                  double2* __restrict__ partial) {
@@ -576,161 +349,6 @@ __global__ void k_setup_finish(SetupParams p, const double* __restrict__ integ, 
   }
 }
 
-// ===================================================== fused path kernels
-// First inverse pass (x axis) with the filter fused: one tile of the resident
-// half spectrum is loaded once and reused for nb consecutive radii; each
-// radius multiplies by W(|k| R_r)/N^3, transforms and writes its own
-// intermediate buffer T_r = T + r_local * N*N*N/2.
-template <int L, int C>
-__global__ void __launch_bounds__(Plan<L>::T* C)
-    k_fx_filter(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
-                float2* __restrict__ Tbuf, int nColBlocks, float dk, float invN3,
-                const RadiusConst* __restrict__ rc, int r0, int nb, const float2* __restrict__ tw) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-  constexpr int N = L;
-  constexpr long long Nh = N / 2;
-  __shared__ float2 smem[L * C];
-  const int t = threadIdx.x;
-  const int c = t % C, i = t / C;
-  const long long blk = xcd_tile(blockIdx.x, gridDim.x);
-  const int cb = (int)(blk % nColBlocks);
-  const int ky = (int)(blk / nColBlocks);
-  const int kz = cb * C + c;
-  const long long sA = (long long)N * Nh;
-  const long long off = (long long)ky * Nh + kz;
-  const bool packed = (kz == 0);
-  LdsPencil lds{smem + c, C};
-  float2 m[E];
-  int msq[E];
-  const int fy = fsq(ky, N);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int kx = i + e * T;
-    msq[e] = fsq(kx, N) + fy;
-    m[e] = packed ? D0[(long long)kx * N + ky] : S[off + (long long)kx * sA];
-  }
-  for (int rl = 0; rl < nb; ++rl) {
-    const float dkR = dk * rc[r0 + rl].R;
-    float2 v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (!packed) {
-        v[e] = cscale(m[e], filter_coef(msq[e] + kz * kz, dkR, invN3));
-      } else {
-        const float w0 = filter_coef(msq[e], dkR, invN3);
-        const float wn = filter_coef(msq[e] + (int)(Nh * Nh), dkR, invN3);
-        const float2 b = DN[(long long)(i + e * T) * N + ky];
-        v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
-      }
-    }
-    pencil_fft<L, +1>(v, i, tw, lds);
-    float2* dst = Tbuf + (long long)rl * N * N * Nh + off;
-#pragma unroll
-    for (int e = 0; e < E; ++e) dst[(long long)(i + e * T) * sA] = v[e];
-  }
-}
-
-// Last inverse pass (z axis, c2r) with the barrier fused: for each row the nb
-// intermediates are transformed in turn, d2z() is evaluated per cell, the
-// maximum over the batch is kept in registers (first radius wins ties) and
-// zreion is read and written once.  Sum of delta_R^2 per radius is kept in
-// FP64 per thread and reduced per block (sigma_grid).
-constexpr int kFusedMaxRadii = 8;
-
-template <int L>
-__global__ void __launch_bounds__(256)
-    k_fz_barrier(const float2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub,
-                 long long nrows, const RadiusConst* __restrict__ rc, int r0, int nb,
-                 double* __restrict__ partial, const float2* __restrict__ tw_half,
-                 const float2* __restrict__ tw_full) {
-  using P = Plan<L>;
-  constexpr int T = P::T, E = P::E;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  __shared__ float2 smem[ROWS * L];
-  __shared__ double red[kFusedMaxRadii][4];
-  const int t = threadIdx.x;
-  const int i = t % T, rr = t / T;
-  const long long plane = (long long)L * nrows;  // complex elements per intermediate
-  LdsPencil lds{smem + rr * L, 1};
-  double s2[kFusedMaxRadii];
-#pragma unroll
-  for (int q = 0; q < kFusedMaxRadii; ++q) s2[q] = 0.0;
-  for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
-    const long long row = base + rr;
-    const bool valid = row < nrows;
-    float2 best[E];
-    float bestR[E], bestR1[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      best[e] = make_float2(-INFINITY, -INFINITY);
-      bestR[e] = 0.f;
-      bestR1[e] = 0.f;
-    }
-#pragma unroll
-    for (int q = 0; q < kFusedMaxRadii; ++q) {
-      if (q < nb) {
-        const float2* rp = Tbuf + (long long)q * plane + (valid ? row : 0) * L;
-        float2 v[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = rp[i + e * T];
-        c2r_preprocess<L>(v, rp, i, tw_full);
-        pencil_fft<L, +1>(v, i, tw_half, lds);
-        const RadiusConst c = rc[r0 + q];
-        double acc = 0.0;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const float z0 = barrier_z(v[e].x, c.A, c.B);
-          const float z1 = barrier_z(v[e].y, c.A, c.B);
-          acc += (double)v[e].x * (double)v[e].x + (double)v[e].y * (double)v[e].y;
-          if (z0 > best[e].x) {
-            best[e].x = z0;
-            bestR[e] = c.R;
-          }
-          if (z1 > best[e].y) {
-            best[e].y = z1;
-            bestR1[e] = c.R;
-          }
-        }
-        if (valid) s2[q] += acc;
-      }
-    }
-    if (valid) {
-      float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
-      float2* bp = rbub ? reinterpret_cast<float2*>(rbub + row * (2 * L)) : nullptr;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int n = i + e * T;
-        float2 old = zp[n];
-        const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
-        if (u0 || u1) {
-          float2 nz = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
-          zp[n] = nz;
-          if (bp) {
-            float2 ob = bp[n];
-            bp[n] = make_float2(u0 ? bestR[e] : ob.x, u1 ? bestR1[e] : ob.y);
-          }
-        }
-      }
-    }
-  }
-  // block reduction of the per-radius sums (fixed order -> deterministic)
-  const int w = t >> 6, l = t & 63;
-#pragma unroll
-  for (int q = 0; q < kFusedMaxRadii; ++q) {
-    if (q < nb) {
-      double s = wave_sum(s2[q]);
-      if (l == 0) red[q][w] = s;
-    }
-  }
-  __syncthreads();
-  if (t < nb) {
-    double s = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[t][k];
-    partial[(long long)(r0 + t) * gridDim.x + blockIdx.x] = s;
-  }
-}
-
 // zero fill (graph-safe replacement for a memset)
 __global__ void k_fill(float* __restrict__ p, long long n, float v) {
   for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
@@ -759,100 +377,6 @@ static inline unsigned grid1(long long n, int bs, long long cap = 1 << 20) {
 
 hipError_t launch_twiddles(float2* tw, int L, hipStream_t s) {
   hipLaunchKernelGGL(k_twiddles, dim3((L + 255) / 256), dim3(256), 0, s, tw, L);
-  return hipGetLastError();
-}
-
-template <int L>
-static hipError_t fwd_z_impl(const float* in, long long in_row_stride, float2* out, long long nrows, float dinit,
-                             const float2* tw_half, const float2* tw_full, hipStream_t s) {
-  constexpr int T = Plan<L>::T;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  hipLaunchKernelGGL(k_fwd_z_r2c<L>, dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(T * ROWS), 0, s, in,
-                     in_row_stride, out, nrows, dinit, tw_half, tw_full);
-  return hipGetLastError();
-}
-
-template <int L>
-static hipError_t inv_z_impl(float2* data, long long nrows, const float2* tw_half, const float2* tw_full,
-                             hipStream_t s) {
-  constexpr int T = Plan<L>::T;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  hipLaunchKernelGGL(k_inv_z_c2r<L>, dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(T * ROWS), 0, s, data,
-                     nrows, tw_half, tw_full);
-  return hipGetLastError();
-}
-
-template <int L, int DIR>
-static hipError_t strided_impl(float2* data, long long sA, long long sO, long long nOuter, int ncols,
-                               const float2* tw, hipStream_t s) {
-  constexpr int C0 = (L >= 2048) ? 8 : 16;
-  if (ncols % C0 == 0) {
-    const int ncb = ncols / C0;
-    hipLaunchKernelGGL((k_strided<L, C0, DIR>), dim3((unsigned)(nOuter * ncb)), dim3(Plan<L>::T * C0), 0, s,
-                       data, sA, sO, ncb, tw);
-  } else if (ncols % 8 == 0) {
-    const int ncb = ncols / 8;
-    hipLaunchKernelGGL((k_strided<L, 8, DIR>), dim3((unsigned)(nOuter * ncb)), dim3(Plan<L>::T * 8), 0, s, data,
-                       sA, sO, ncb, tw);
-  } else {
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-#define PRC_FOR_EACH_HALF(X) X(8) X(16) X(32) X(64) X(96) X(128) X(256) X(512) X(1024)
-#define PRC_FOR_EACH_FULL(X) X(16) X(32) X(64) X(128) X(192) X(256) X(512) X(1024) X(2048)
-
-hipError_t launch_fwd_z(int Nh, const float* in, long long in_row_stride, float2* out, long long nrows,
-                        float dinit, const float2* tw_half, const float2* tw_full, hipStream_t s) {
-  switch (Nh) {
-#define X(LL) \
-  case LL:    \
-    return fwd_z_impl<LL>(in, in_row_stride, out, nrows, dinit, tw_half, tw_full, s);
-    PRC_FOR_EACH_HALF(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
-  }
-}
-
-hipError_t launch_inv_z(int Nh, float2* data, long long nrows, const float2* tw_half, const float2* tw_full,
-                        hipStream_t s) {
-  switch (Nh) {
-#define X(LL) \
-  case LL:    \
-    return inv_z_impl<LL>(data, nrows, tw_half, tw_full, s);
-    PRC_FOR_EACH_HALF(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
-  }
-}
-
-hipError_t launch_strided(int L, int dir, float2* data, long long sA, long long sO, long long nOuter, int ncols,
-                          const float2* tw, hipStream_t s) {
-  switch (L) {
-#define X(LL)                                                                   \
-  case LL:                                                                      \
-    return dir > 0 ? strided_impl<LL, +1>(data, sA, sO, nOuter, ncols, tw, s) \
-                   : strided_impl<LL, -1>(data, sA, sO, nOuter, ncols, tw, s);
-    PRC_FOR_EACH_FULL(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
-  }
-}
-
-hipError_t launch_unpack(const float2* S, float2* D0, float2* DN, int N, hipStream_t s) {
-  const long long n = (long long)N * N;
-  hipLaunchKernelGGL(k_unpack_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, D0, DN, N);
-  return hipGetLastError();
-}
-
-hipError_t launch_filter(const float2* S, const float2* D0, const float2* DN, float2* out, int N, float dkR,
-                         float invN3, hipStream_t s) {
-  const long long n = (long long)N * N * (N / 2);
-  hipLaunchKernelGGL(k_filter, dim3(grid1(n, 256)), dim3(256), 0, s, S, D0, DN, out, N, dkR, invN3);
   return hipGetLastError();
 }
 
@@ -886,75 +410,6 @@ hipError_t launch_setup(const SetupParams& p, double* integ, float* Rr, double* 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_setup_finish, dim3(1), dim3(64), 0, s, p, integ, Rr, sc, rc, sigma_out);
   return hipGetLastError();
-}
-
-template <int L>
-static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, float2* Tbuf, int N, float dk,
-                          float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
-  constexpr int C = (L >= 512) ? 8 : ((L / 2) >= 16 ? 16 : 8);
-  const int Nh = N / 2;
-  if (Nh % C != 0) return hipErrorInvalidValue;
-  const int ncb = Nh / C;
-  hipLaunchKernelGGL((k_fx_filter<L, C>), dim3((unsigned)(N * ncb)), dim3(Plan<L>::T * C), 0, s, S, D0, DN, Tbuf,
-                     ncb, dk, invN3, rc, r0, nb, tw);
-  return hipGetLastError();
-}
-
-hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
-                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
-  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
-  switch (N) {
-#define X(LL) \
-  case LL:    \
-    return fx_impl<LL>(S, D0, DN, Tbuf, N, dk, invN3, rc, r0, nb, tw, s);
-    PRC_FOR_EACH_FULL(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
-  }
-}
-
-template <int L>
-static hipError_t fz_impl(const float2* Tbuf, float* zre, float* rbub, long long nrows, const RadiusConst* rc,
-                          int r0, int nb, double* partial, int nblk, const float2* tw_half, const float2* tw_full,
-                          hipStream_t s) {
-  constexpr int T = Plan<L>::T;
-  constexpr int ROWS = (256 / T) > 0 ? 256 / T : 1;
-  hipLaunchKernelGGL(k_fz_barrier<L>, dim3(nblk), dim3(T * ROWS), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
-                     partial, tw_half, tw_full);
-  return hipGetLastError();
-}
-
-int fz_blocks(int Nh, long long nrows) {
-  int T = 1;
-  switch (Nh) {
-#define X(LL)          \
-  case LL:             \
-    T = Plan<LL>::T;   \
-    break;
-    PRC_FOR_EACH_HALF(X)
-#undef X
-    default:
-      return 0;
-  }
-  const int rows = (256 / T) > 0 ? 256 / T : 1;
-  long long need = (nrows + rows - 1) / rows;
-  return (int)(need < 4096 ? need : 4096);
-}
-
-hipError_t launch_fz(int Nh, const float2* Tbuf, float* zre, float* rbub, long long nrows, const RadiusConst* rc,
-                     int r0, int nb, double* partial, int nblk, const float2* tw_half, const float2* tw_full,
-                     hipStream_t s) {
-  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
-  switch (Nh) {
-#define X(LL) \
-  case LL:    \
-    return fz_impl<LL>(Tbuf, zre, rbub, nrows, rc, r0, nb, partial, nblk, tw_half, tw_full, s);
-    PRC_FOR_EACH_HALF(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
-  }
 }
 
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s) {

@@ -186,33 +186,41 @@ struct DFT {
 };
 
 // --------------------------------------------------------------- plans
-// Per transform length: E elements per thread and the radix sequence.
+// A plan: transform length L, E elements per thread (T = L/E threads per
+// pencil) and the radix sequence; every radix divides E.
+template <int L_, int E_, int... R>
+struct PlanT {
+  static constexpr int L = L_, E = E_, T = L_ / E_;
+  static constexpr int radices[] = {R...};
+  static constexpr int S = sizeof...(R);
+};
+
+// Default plans (one 64-lane wave or less per pencil for the z passes).
 template <int L>
 struct Plan;
-#define PRC_PLAN(LEN, EE, ...)                                    \
-  template <>                                                     \
-  struct Plan<LEN> {                                              \
-    static constexpr int L = LEN, E = EE, T = LEN / EE;           \
-    static constexpr int radices[] = {__VA_ARGS__};               \
-    static constexpr int S = sizeof(radices) / sizeof(int);       \
-  };
-PRC_PLAN(8, 8, 8)
-PRC_PLAN(16, 16, 16)
-PRC_PLAN(32, 8, 8, 4)
-PRC_PLAN(64, 8, 8, 8)
-PRC_PLAN(96, 24, 3, 8, 4)
-PRC_PLAN(128, 16, 16, 8)
-PRC_PLAN(192, 24, 3, 8, 8)
-PRC_PLAN(256, 16, 16, 16)
-PRC_PLAN(512, 8, 8, 8, 8)
-PRC_PLAN(1024, 16, 16, 16, 4)
-PRC_PLAN(2048, 16, 16, 16, 8)
-#undef PRC_PLAN
+template <> struct Plan<8> : PlanT<8, 8, 8> {};
+template <> struct Plan<16> : PlanT<16, 16, 16> {};
+template <> struct Plan<32> : PlanT<32, 8, 8, 4> {};
+template <> struct Plan<64> : PlanT<64, 8, 8, 8> {};
+template <> struct Plan<96> : PlanT<96, 24, 3, 8, 4> {};
+template <> struct Plan<128> : PlanT<128, 16, 16, 8> {};
+template <> struct Plan<192> : PlanT<192, 24, 3, 8, 8> {};
+template <> struct Plan<256> : PlanT<256, 16, 16, 16> {};
+template <> struct Plan<512> : PlanT<512, 8, 8, 8, 8> {};
+template <> struct Plan<1024> : PlanT<1024, 16, 16, 16, 4> {};
+template <> struct Plan<2048> : PlanT<2048, 16, 16, 16, 8> {};
 
-template <int L, int S>
+// Wider plans for kernels that keep a second register copy of their tile
+// (the fused filter pass): fewer threads per pencil, more registers each.
+template <int L>
+struct PlanWide : Plan<L> {};
+template <> struct PlanWide<1024> : PlanT<1024, 32, 32, 32> {};
+template <> struct PlanWide<2048> : PlanT<2048, 32, 32, 16, 4> {};
+
+template <class P, int S>
 constexpr int plan_p() {  // product of the first S radices
   int p = 1;
-  for (int s = 0; s < S; ++s) p *= Plan<L>::radices[s];
+  for (int s = 0; s < S; ++s) p *= P::radices[s];
   return p;
 }
 
@@ -226,15 +234,15 @@ struct LdsPencil {
 // One Stockham stage on registers v (thread index i within the pencil).
 // If LAST, outputs stay in registers (natural order, v[e] = y[i + e*T]);
 // otherwise they are written to LDS at their Stockham positions.
-template <int L, int STAGE, int DIR, bool LAST>
-__device__ __forceinline__ void stockham_stage(float2 (&v)[Plan<L>::E], int i, const float2* __restrict__ tw,
+template <class P, int STAGE, int DIR, bool LAST>
+__device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const float2* __restrict__ tw,
                                                LdsPencil lds) {
-  using P = Plan<L>;
-  constexpr int E = P::E, T = P::T;
+  constexpr int L = P::L, E = P::E, T = P::T;
   constexpr int R = P::radices[STAGE];
-  constexpr int p = plan_p<L, STAGE>();
-  constexpr int NB = E / R;      // butterflies per thread
+  constexpr int p = plan_p<P, STAGE>();
+  constexpr int NB = E / R;         // butterflies per thread
   constexpr int TWS = L / (p * R);  // twiddle table stride
+  static_assert(E % R == 0, "radix must divide E");
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     float2 u[R];
@@ -262,27 +270,46 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[Plan<L>::E], int i, c
   }
 }
 
-template <int L>
-__device__ __forceinline__ void lds_gather(float2 (&v)[Plan<L>::E], int i, LdsPencil lds) {
-  using P = Plan<L>;
+template <class P>
+__device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, LdsPencil lds) {
 #pragma unroll
   for (int e = 0; e < P::E; ++e) v[e] = lds[i + e * P::T];
 }
 
 // Full pencil transform: registers in natural layout -> registers in natural
-// layout.  All threads of the block must call it (it contains barriers).
-template <int L, int DIR, int STAGE = 0>
-__device__ __forceinline__ void pencil_fft(float2 (&v)[Plan<L>::E], int i, const float2* __restrict__ tw,
+// layout.  All threads of the block must call it (it contains barriers); the
+// LDS region may be reused right after it returns.
+template <class P, int DIR, int STAGE = 0>
+__device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw,
                                            LdsPencil lds) {
-  using P = Plan<L>;
   if constexpr (STAGE == P::S - 1) {
-    stockham_stage<L, STAGE, DIR, true>(v, i, tw, lds);
+    stockham_stage<P, STAGE, DIR, true>(v, i, tw, lds);
   } else {
-    stockham_stage<L, STAGE, DIR, false>(v, i, tw, lds);
+    stockham_stage<P, STAGE, DIR, false>(v, i, tw, lds);
     __syncthreads();
-    lds_gather<L>(v, i, lds);
+    lds_gather<P>(v, i, lds);
     __syncthreads();
-    pencil_fft<L, DIR, STAGE + 1>(v, i, tw, lds);
+    pencil_fft<P, DIR, STAGE + 1>(v, i, tw, lds);
+  }
+}
+
+// Mirror element x[(L - k) mod L] of every register x[k], k = i + e*T, for
+// pencils held by T <= 64 consecutive lanes of a wave: for i > 0 it sits in
+// lane (T - i) of the pencil at register E-1-e, for i = 0 in this lane at
+// register (E - e) mod E.  Cross-lane moves use ds_bpermute (no LDS storage).
+template <class P>
+__device__ __forceinline__ void mirror_lanes(const float2 (&v)[P::E], float2 (&m)[P::E], int i) {
+  constexpr int E = P::E, T = P::T;
+  static_assert(T <= 64 && 64 % T == 0, "pencil must fit in one wave");
+  const int lane = __lane_id();
+  const int src = (lane - i) + ((T - i) % T);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float2 w = v[E - 1 - e];
+    float2 got;
+    got.x = __shfl(w.x, src, 64);
+    got.y = __shfl(w.y, src, 64);
+    m[e] = (i == 0) ? v[(E - e) % E] : got;
   }
 }
 

@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "d2z_device.h"
 #include "d2z_kernels.h"
 #include "fft_core.h"
 
@@ -104,13 +105,9 @@ __device__ __forceinline__ float grf_amp(int m, const GrfSpectrum& sp, double A)
   return (float)(sqrt(P / sp.vcell) * sp.invN3);
 }
 
-__device__ __forceinline__ int fsq_g(int i, int N) {
-  int f = (i >= (N >> 1)) ? i - N : i;
-  return f * f;
-}
-
-// colour the half spectrum in place; the kz = 0 slot is rebuilt from the side
-// planes as a0 D0 + i aN DN (the packed pair of the z transform)
+// colour the half spectrum S in place; the kz = 0 slot is rebuilt from the
+// side planes as a0 D0 + i aN DN (the packed pair of the z transform)
+template <int C>
 __global__ void k_colour(float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
                          int N, GrfSpectrum sp, const double* __restrict__ Aptr) {
   const double A = Aptr[0];
@@ -118,15 +115,14 @@ __global__ void k_colour(float2* __restrict__ S, const float2* __restrict__ D0, 
   const long long total = (long long)N * N * Nh;
   for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (long long)gridDim.x * blockDim.x) {
-    const int kz = (int)(id % Nh);
-    const long long pl = id / Nh;
-    const int ky = (int)(pl % N), kx = (int)(pl / N);
-    const int mxy = fsq_g(kx, N) + fsq_g(ky, N);
-    if (kz != 0) {
-      const float a = grf_amp(mxy + kz * kz, sp, A);
+    const SIndex k = s_index<C>(id, N);
+    const int mxy = fsq(k.kx, N) + fsq(k.ky, N);
+    if (k.kz != 0) {
+      const float a = grf_amp(mxy + k.kz * k.kz, sp, A);
       const float2 v = S[id];
       S[id] = make_float2(v.x * a, v.y * a);
     } else {
+      const long long pl = (long long)k.kx * N + k.ky;
       const float a0 = grf_amp(mxy, sp, A), an = grf_amp(mxy + (int)(Nh * Nh), sp, A);
       const float2 p = D0[pl], q = DN[pl];
       S[id] = make_float2(a0 * p.x - an * q.y, a0 * p.y + an * q.x);
@@ -146,12 +142,18 @@ hipError_t launch_grf_norm(const GrfSpectrum& sp, double* A, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_colour(float2* S, const float2* D0, const float2* DN, int N, const GrfSpectrum& sp,
+hipError_t launch_colour(int N, float2* S, const float2* D0, const float2* DN, const GrfSpectrum& sp,
                          const double* A, hipStream_t s) {
-  long long n = (long long)N * N * (N / 2);
+  const long long n = (long long)N * N * (N / 2);
   long long g = (n + 255) / 256;
   if (g > (1 << 20)) g = 1 << 20;
-  hipLaunchKernelGGL(k_colour, dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, N, sp, A);
+  const int C = block_width(N);
+  if (C == 16)
+    hipLaunchKernelGGL((k_colour<16>), dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, N, sp, A);
+  else if (C == 8)
+    hipLaunchKernelGGL((k_colour<8>), dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, N, sp, A);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -1,0 +1,589 @@
+// Pencil-pass kernels of the delta2zreion path (gfx950).
+//
+// Layouts (complex elements, C = Blk<N>::C, Nh = N/2, blocked as described in
+// d2z_device.h):
+//   S   spectrum      [kz/C][ky][kx][kz%C]   written once by the forward passes
+//   T_r intermediate  [kz/C][x][y][kz%C]     one per radius of a batch
+//   real fields       dense [x][y][z]
+// The kz = 0 slot of every row holds the packed pair H(kz=0) + i H(kz=N/2) of
+// the z transform (both real along z); the true planes kz = 0 and kz = N/2 of
+// the spectrum are kept unpacked in D0, DN [kx][ky].
+//
+// Passes (reference: main_d2z.C:57-67 forward, delta2zreion_d2z.C:58-146 per
+// radius):
+//   forward  z r2c (rows -> T layout), y (in place), x (T layout -> S)
+//   inverse  x with the filter fused (S -> T_r), y (in place), z c2r with the
+//            barrier and running maximum fused (T_r -> zreion)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "d2z_device.h"
+#include "d2z_kernels.h"
+#include "fft_core.h"
+
+namespace prc {
+
+// ---------------------------------------------------- real-transform glue
+// Forward untangling of Z = FFT_L(x[2n] + i x[2n+1]) into the packed half
+// spectrum: X[k] = (Z[k] + conj Z[L-k])/2 - i exp(-2 pi i k/N) (Z[k] - conj Z[L-k])/2,
+// slot 0 = X[0] + i X[N/2] = (Re Z0 + Im Z0) + i (Re Z0 - Im Z0).
+template <class P>
+__device__ __forceinline__ void r2c_post(float2 (&v)[P::E], const float2 (&m)[P::E], int i,
+                                         const float2* __restrict__ tw_full) {
+#pragma unroll
+  for (int e = 0; e < P::E; ++e) {
+    const int k = i + e * P::T;
+    const float2 Z = v[e];
+    if (k == 0) {
+      v[e] = make_float2(Z.x + Z.y, Z.x - Z.y);
+    } else {
+      const float2 Zm = cconj(m[e]);
+      const float2 Ev = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y + Zm.y));
+      const float2 D = make_float2(0.5f * (Z.x - Zm.x), 0.5f * (Z.y - Zm.y));
+      const float2 Od = make_float2(D.y, -D.x);  // -i D
+      float2 w = tw_full[k];
+      w.y = -w.y;
+      v[e] = cadd(Ev, cmul(w, Od));
+    }
+  }
+}
+
+// Inverse: Z[k] = X[k] + conj X[L-k] + i exp(+2 pi i k/N) (X[k] - conj X[L-k]),
+// Z[0] = (X0 + XN) + i (X0 - XN); then z = IFFT_L(Z) holds x[2n] + i x[2n+1].
+template <class P>
+__device__ __forceinline__ void c2r_pre(float2 (&v)[P::E], const float2 (&m)[P::E], int i,
+                                        const float2* __restrict__ tw_full) {
+#pragma unroll
+  for (int e = 0; e < P::E; ++e) {
+    const int k = i + e * P::T;
+    const float2 a = v[e];
+    if (k == 0) {
+      v[e] = make_float2(a.x + a.y, a.x - a.y);
+    } else {
+      const float2 b = cconj(m[e]);
+      const float2 s = cadd(a, b), d = csub(a, b);
+      const float2 wd = cmul(tw_full[k], d);
+      v[e] = make_float2(s.x - wd.y, s.y + wd.x);
+    }
+  }
+}
+
+// ------------------------------------------------- forward z pass (r2c)
+// Dense real rows (x, y) -> packed half spectrum rows in the T layout.
+template <class P, int C>
+__global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, float2* __restrict__ out,
+                                               long long nrows, float dinit, const float2* __restrict__ tw_half,
+                                               const float2* __restrict__ tw_full) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  constexpr int ROWS = 256 / T;
+  __shared__ float2 smem[ROWS * L];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const long long row = (long long)blockIdx.x * ROWS + rr;
+  const bool valid = row < nrows;
+  const long long r = valid ? row : 0;
+  const float2* src = reinterpret_cast<const float2*>(in + r * (2 * L));
+  float2 v[E], m[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float2 a = src[i + e * T];
+    v[e] = make_float2(a.x / dinit, a.y / dinit);  // delta /= DInit (main_d2z.C:57-59)
+  }
+  pencil_fft<P, -1>(v, i, tw_half, LdsPencil{smem + rr * L, 1});
+  mirror_lanes<P>(v, m, i);
+  r2c_post<P>(v, m, i, tw_full);
+  if (valid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = i + e * T;
+      out[((long long)(k / C) * nrows + row) * C + (k % C)] = v[e];
+    }
+  }
+}
+
+// ------------------------------------------------- tiled complex pass
+// One block = one tile of C columns x L points along the transformed axis.
+// Tile t -> (o, cb) = (t mod nOuter, t / nOuter); element (a, c) is read at
+// cb*sCB + o*in_sO + a*in_sA + c and written at cb*sCB + o*out_sO + a*out_sA + c
+// (in place when in == out with equal strides).
+template <class P, int C, int DIR>
+__global__ void __launch_bounds__(P::T* C)
+    k_tile(const float2* in, float2* out, long long in_sA, long long in_sO, long long out_sA, long long out_sO,
+           long long sCB, int nOuter, const float2* __restrict__ tw) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  __shared__ float2 smem[L * C];
+  const int tid = threadIdx.x;
+  const int c = tid % C, i = tid / C;
+  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long long o = tile % nOuter, cb = tile / nOuter;
+  const float2* ib = in + cb * sCB + o * in_sO + c;
+  float2* ob = out + cb * sCB + o * out_sO + c;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ib[(long long)(i + e * T) * in_sA];
+  pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
+#pragma unroll
+  for (int e = 0; e < E; ++e) ob[(long long)(i + e * T) * out_sA] = v[e];
+}
+
+// ---------------------------------------------------- inverse z pass (c2r)
+// Rows (x, y) gathered from the T layout -> dense real rows.
+template <class P, int C>
+__global__ void __launch_bounds__(256) k_inv_z(const float2* __restrict__ in, float* __restrict__ out,
+                                               long long nrows, const float2* __restrict__ tw_half,
+                                               const float2* __restrict__ tw_full) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  constexpr int ROWS = 256 / T;
+  __shared__ float2 smem[ROWS * L];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const long long row = (long long)blockIdx.x * ROWS + rr;
+  const bool valid = row < nrows;
+  const long long r = valid ? row : 0;
+  float2 v[E], m[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = i + e * T;
+    v[e] = in[((long long)(k / C) * nrows + r) * C + (k % C)];
+  }
+  mirror_lanes<P>(v, m, i);
+  c2r_pre<P>(v, m, i, tw_full);
+  pencil_fft<P, +1>(v, i, tw_half, LdsPencil{smem + rr * L, 1});
+  if (valid) {
+    float2* dst = reinterpret_cast<float2*>(out + row * (2 * L));
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[i + e * T] = v[e];
+  }
+}
+
+// ------------------------------------------------- unpack after forward
+// D0(k) = (P(k) + conj P(-k))/2,  DN(k) = -i (P(k) - conj P(-k))/2 with P the
+// packed kz = 0 slot of S; D0/DN indexed [kx][ky].
+template <int C>
+__global__ void k_unpack(const float2* __restrict__ S, float2* __restrict__ D0, float2* __restrict__ DN, int N) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long long)N * N) return;
+  const int kx = (int)(id / N), ky = (int)(id % N);
+  const int mx = (N - kx) % N, my = (N - ky) % N;
+  const float2 p = S[((long long)ky * N + kx) * C];
+  const float2 q = cconj(S[((long long)my * N + mx) * C]);
+  D0[id] = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y + q.y));
+  const float2 d = make_float2(0.5f * (p.x - q.x), 0.5f * (p.y - q.y));
+  DN[id] = make_float2(d.y, -d.x);
+}
+
+// ------------------------------------------------ plain path: filter
+// T = W(|k| R)/N^3 * S elementwise; the kz = 0 slot rebuilt as W0 D0 + i WN DN
+// (delta2zreion_d2z.C:58-89 with NormalizeArray's 1/N^3, :93).
+template <int C>
+__global__ void k_filter(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+                         float2* __restrict__ out, int N, float dkR, float invN3) {
+  const long long Nh = N / 2;
+  const long long total = (long long)N * N * Nh;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const SIndex k = s_index<C>(id, N);
+    const int mxy = fsq(k.kx, N) + fsq(k.ky, N);
+    float2 r;
+    if (k.kz != 0) {
+      r = cscale(S[id], filter_coef(mxy + k.kz * k.kz, dkR, invN3));
+    } else {
+      const long long pl = (long long)k.kx * N + k.ky;
+      const float w0 = filter_coef(mxy, dkR, invN3);
+      const float wn = filter_coef(mxy + (int)(Nh * Nh), dkR, invN3);
+      const float2 a = D0[pl], b = DN[pl];
+      r = make_float2(w0 * a.x - wn * b.y, w0 * a.y + wn * b.x);
+    }
+    out[id] = r;
+  }
+}
+
+// ============================================================ fused path
+// First inverse pass (x) with the filter fused.  A block owns CF of the C
+// columns of one spectrum tile (ky, kz block); sibling sub-tiles are adjacent
+// in dispatch order, so their partial lines meet in the same L2.  The
+// spectrum values are read once into registers and reused for nb radii: each
+// radius scales by W(|k| R_r)/N^3, transforms along x and writes its own
+// intermediate T_r = Tbuf + r_local * N*N*N/2 (T layout).
+template <class P, int C, int CF>
+__global__ void __launch_bounds__(P::T* CF)
+    k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+         float2* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
+         const float2* __restrict__ tw) {
+  constexpr int N = P::L, T = P::T, E = P::E;
+  constexpr int NSUB = C / CF;
+  constexpr long long Nh = N / 2;
+  constexpr long long plane = (long long)N * N * Nh;
+  __shared__ float2 smem[N * CF];
+  const int tid = threadIdx.x;
+  const int cl = tid % CF, i = tid / CF;
+  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int sub = (int)(tile % NSUB);
+  const int ky = (int)((tile / NSUB) % N);
+  const int kzb = (int)(tile / (NSUB * N));
+  const int c = sub * CF + cl;
+  const int kz = kzb * C + c;
+  const bool packed = (kz == 0);
+  const float2* sp = S + ((long long)kzb * N + ky) * N * C + c;  // element kx at sp[kx*C]
+  float2 m[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int kx = i + e * T;
+    m[e] = packed ? D0[(long long)kx * N + ky] : sp[(long long)kx * C];
+  }
+  const int base2 = fsq(ky, N) + kz * kz;
+  float2* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + c;  // element x at tp[x*N*C]
+  const LdsPencil lds{smem + cl, CF};
+#pragma unroll 1
+  for (int rl = 0; rl < nb; ++rl) {
+    const float dkR = dk * rc[r0 + rl].R;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = i + e * T;
+      const int msq = fsq(kx, N) + base2;
+      if (!packed) {
+        v[e] = cscale(m[e], filter_coef(msq, dkR, invN3));
+      } else {
+        const float w0 = filter_coef(msq, dkR, invN3);
+        const float wn = filter_coef(msq + (int)(Nh * Nh), dkR, invN3);
+        const float2 b = DN[(long long)kx * N + ky];
+        v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
+      }
+    }
+    pencil_fft<P, +1>(v, i, tw, lds);
+    float2* dst = tp + (long long)rl * plane;
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[(long long)(i + e * T) * N * C] = v[e];
+  }
+}
+
+// columns per fused-x block: keep the block at <= 512 threads
+template <int N>
+struct FxCols {
+  static constexpr int C = Blk<N>::C;
+  static constexpr int CF = (Plan<N>::T * C <= 512) ? C : (Plan<N>::T * (C / 2) <= 512 ? C / 2 : C / 4);
+};
+
+// Last inverse pass (z, c2r) with the barrier fused: for each row the nb
+// intermediates are transformed in turn, d2z() is evaluated per cell, the
+// maximum over the batch stays in registers (first radius wins ties) and
+// zreion (and rbubble) is read and written once per batch.  Per-radius sums of
+// delta_R^2 (float products, double sums as GetMeanSigma) are kept per wave in
+// LDS and written as one partial per block and radius.
+template <class P, int C, bool RB>
+__global__ void __launch_bounds__(256)
+    k_fz(const float2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
+         const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial,
+         const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  constexpr int ROWS = 256 / T;
+  __shared__ float2 smem[ROWS * L];
+  __shared__ double red[4][kFusedMaxRadii];
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const int w = t >> 6, lane = t & 63;
+  if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
+  __syncthreads();
+  const long long plane = nrows * L;
+  const LdsPencil lds{smem + rr * L, 1};
+#pragma unroll 1
+  for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
+    const long long row = base + rr;
+    const bool valid = row < nrows;
+    const long long r = valid ? row : 0;
+    float2 best[E];
+    float bR0[E], bR1[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      best[e] = make_float2(-INFINITY, -INFINITY);
+      bR0[e] = 0.f;
+      bR1[e] = 0.f;
+    }
+#pragma unroll 1
+    for (int q = 0; q < nb; ++q) {
+      const float2* tp = Tbuf + (long long)q * plane;
+      float2 v[E], m[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = i + e * T;
+        v[e] = tp[((long long)(k / C) * nrows + r) * C + (k % C)];
+      }
+      mirror_lanes<P>(v, m, i);
+      c2r_pre<P>(v, m, i, tw_full);
+      pencil_fft<P, +1>(v, i, tw_half, lds);
+      const double A = rc[r0 + q].A, B = rc[r0 + q].B;
+      const float R = rc[r0 + q].R;
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float z0 = barrier_z(v[e].x, A, B);
+        const float z1 = barrier_z(v[e].y, A, B);
+        acc += (double)(v[e].x * v[e].x) + (double)(v[e].y * v[e].y);
+        if (z0 > best[e].x) {
+          best[e].x = z0;
+          if (RB) bR0[e] = R;
+        }
+        if (z1 > best[e].y) {
+          best[e].y = z1;
+          if (RB) bR1[e] = R;
+        }
+      }
+      acc = wave_sum(valid ? acc : 0.0);
+      if (lane == 0) red[w][q] += acc;
+    }
+    if (valid) {
+      float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
+      float2* bp = RB ? reinterpret_cast<float2*>(rbub + row * (2 * L)) : nullptr;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int n = i + e * T;
+        const float2 old = zp[n];
+        const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
+        if (u0 || u1) {
+          zp[n] = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
+          if (RB) {
+            const float2 ob = bp[n];
+            bp[n] = make_float2(u0 ? bR0[e] : ob.x, u1 ? bR1[e] : ob.y);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (t < nb) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k][t];
+    partial[(long long)(r0 + t) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// ============================================================ launchers
+#define PRC_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(192) X(256) X(512) X(1024) X(2048)
+
+template <int N>
+static hipError_t fwd_z_impl(const float* in, float2* out, float dinit, const float2* twh, const float2* twf,
+                             hipStream_t s) {
+  using P = Plan<N / 2>;
+  constexpr int ROWS = 256 / P::T;
+  const long long nrows = (long long)N * N;
+  hipLaunchKernelGGL((k_fwd_z<P, Blk<N>::C>), dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(256), 0, s, in,
+                     out, nrows, dinit, twh, twf);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t inv_z_impl(const float2* in, float* out, const float2* twh, const float2* twf, hipStream_t s) {
+  using P = Plan<N / 2>;
+  constexpr int ROWS = 256 / P::T;
+  const long long nrows = (long long)N * N;
+  hipLaunchKernelGGL((k_inv_z<P, Blk<N>::C>), dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(256), 0, s, in,
+                     out, nrows, twh, twf);
+  return hipGetLastError();
+}
+
+template <int N, int DIR>
+static hipError_t tile_impl(const float2* in, float2* out, const TileMap& mp, const float2* tw, hipStream_t s) {
+  using P = Plan<N>;
+  constexpr int C = Blk<N>::C;
+  const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
+  hipLaunchKernelGGL((k_tile<P, C, DIR>), dim3((unsigned)ntiles), dim3(P::T * C), 0, s, in, out, mp.in_sA,
+                     mp.in_sO, mp.out_sA, mp.out_sO, mp.sCB, mp.nOuter, tw);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t unpack_impl(const float2* S, float2* D0, float2* DN, hipStream_t s) {
+  const long long n = (long long)N * N;
+  hipLaunchKernelGGL((k_unpack<Blk<N>::C>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, D0, DN, N);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t filter_impl(const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
+                              float invN3, hipStream_t s) {
+  const long long n = (long long)N * N * (N / 2);
+  long long g = (n + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  hipLaunchKernelGGL((k_filter<Blk<N>::C>), dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, out, N, dkR, invN3);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
+                          const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+  using P = Plan<N>;
+  constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
+  const long long nblk = (long long)N * ((N / 2) / C) * (C / CF);
+  hipLaunchKernelGGL((k_fx<P, C, CF>), dim3((unsigned)nblk), dim3(P::T * CF), 0, s, S, D0, DN, Tbuf, dk, invN3, rc,
+                     r0, nb, tw);
+  return hipGetLastError();
+}
+
+template <int N>
+static int fz_grid(long long nrows) {
+  constexpr int ROWS = 256 / Plan<N / 2>::T;
+  const long long need = (nrows + ROWS - 1) / ROWS;
+  return (int)(need < 2048 ? need : 2048);
+}
+
+template <int N>
+static hipError_t fz_impl(const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                          double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s) {
+  using P = Plan<N / 2>;
+  constexpr int C = Blk<N>::C;
+  const long long nrows = (long long)N * N;
+  if (rbub)
+    hipLaunchKernelGGL((k_fz<P, C, true>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb, partial,
+                       twh, twf);
+  else
+    hipLaunchKernelGGL((k_fz<P, C, false>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
+                       partial, twh, twf);
+  return hipGetLastError();
+}
+
+int block_width(int N) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return Blk<NN>::C;
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+TileMap tile_map(int N, Axis axis) {
+  const int C = block_width(N);
+  const long long NC = (long long)N * C;
+  TileMap m{};
+  m.nOuter = N;
+  m.nColBlocks = (N / 2) / C;
+  m.sCB = (long long)N * N * C;
+  switch (axis) {
+    case Axis::kYInPlace:  // T layout (x, y): tile (x, kz block), transform along y
+      m.in_sA = m.out_sA = C;
+      m.in_sO = m.out_sO = NC;
+      break;
+    case Axis::kXTtoS:  // forward x: T layout (x, ky) -> S layout (ky, kx)
+      m.in_sA = NC;
+      m.in_sO = C;
+      m.out_sA = C;
+      m.out_sO = NC;
+      break;
+    case Axis::kXStoT:  // inverse x: S layout (ky, kx) -> T layout (x, ky)
+      m.in_sA = C;
+      m.in_sO = NC;
+      m.out_sA = NC;
+      m.out_sO = C;
+      break;
+  }
+  return m;
+}
+
+hipError_t launch_fwd_z(int N, const float* in, float2* out, float dinit, const float2* twh, const float2* twf,
+                        hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return fwd_z_impl<NN>(in, out, dinit, twh, twf, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_inv_z(int N, const float2* in, float* out, const float2* twh, const float2* twf, hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return inv_z_impl<NN>(in, out, twh, twf, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const TileMap& mp, const float2* tw,
+                       hipStream_t s) {
+  switch (N) {
+#define X(NN)                                                       \
+  case NN:                                                          \
+    return dir > 0 ? tile_impl<NN, +1>(in, out, mp, tw, s) : tile_impl<NN, -1>(in, out, mp, tw, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_unpack(int N, const float2* S, float2* D0, float2* DN, hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return unpack_impl<NN>(S, D0, DN, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
+                         float invN3, hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return filter_impl<NN>(S, D0, DN, out, dkR, invN3, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
+                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return fx_impl<NN>(S, D0, DN, Tbuf, dk, invN3, rc, r0, nb, tw, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int fz_blocks(int N) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return fz_grid<NN>((long long)NN * NN);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+hipError_t launch_fz(int N, const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                     double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s) {
+  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return fz_impl<NN>(Tbuf, zre, rbub, rc, r0, nb, partial, nblk, twh, twf, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace prc

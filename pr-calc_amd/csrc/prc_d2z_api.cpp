@@ -55,10 +55,10 @@ struct KernelTimer {
     used = 0;
   }
   void destroy() {
-    for (auto e : pool) hipEventDestroy(e);
+    for (auto e : pool) (void)hipEventDestroy(e);
     pool.clear();
-    if (t0) hipEventDestroy(t0);
-    if (t1) hipEventDestroy(t1);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
     t0 = t1 = nullptr;
   }
 };
@@ -87,7 +87,8 @@ struct prc_d2z_ctx {
   float2* d_D0 = nullptr;
   float2* d_DN = nullptr;
   float2* d_T = nullptr;
-  int nbuf = 1;
+  int nbuf = 2;    // intermediate buffers held
+  int nbatch = 1;  // radii per fused batch
   float* d_zre = nullptr;
   float* d_rb = nullptr;
   float2* d_tw_full = nullptr;
@@ -158,22 +159,19 @@ hipError_t timed(prc_d2z_ctx* c, const char* name, F&& f) {
   } while (0)
 
 int forward(prc_d2z_ctx* c, float dinit) {
-  const int N = c->N, Nh = c->Nh;
+  const int N = c->N;
   hipStream_t s = c->stream;
+  float2* T0 = c->d_T;
+  const TileMap ym = tile_map(N, Axis::kYInPlace), xm = tile_map(N, Axis::kXTtoS);
   PRC_CHECK(c, timed(c, "fwd_z_r2c", [&] {
-              return launch_fwd_z(Nh, c->d_delta, N, c->d_S, (long long)N * N, dinit, c->d_tw_half,
-                                  c->d_tw_full, s);
+              return launch_fwd_z(N, c->d_delta, T0, dinit, c->d_tw_half, c->d_tw_full, s);
             }),
             "forward z pass");
-  PRC_CHECK(c, timed(c, "fwd_y", [&] {
-              return launch_strided(N, -1, c->d_S, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s);
-            }),
+  PRC_CHECK(c, timed(c, "fwd_y", [&] { return launch_tile(N, -1, T0, T0, ym, c->d_tw_full, s); }),
             "forward y pass");
-  PRC_CHECK(c, timed(c, "fwd_x", [&] {
-              return launch_strided(N, -1, c->d_S, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s);
-            }),
+  PRC_CHECK(c, timed(c, "fwd_x", [&] { return launch_tile(N, -1, T0, c->d_S, xm, c->d_tw_full, s); }),
             "forward x pass");
-  PRC_CHECK(c, timed(c, "unpack", [&] { return launch_unpack(c->d_S, c->d_D0, c->d_DN, N, s); }), "unpack");
+  PRC_CHECK(c, timed(c, "unpack", [&] { return launch_unpack(N, c->d_S, c->d_D0, c->d_DN, s); }), "unpack");
   c->have_spectrum = true;
   return PRC_D2Z_OK;
 }
@@ -184,30 +182,31 @@ float host_dk(const prc_d2z_ctx* c) {
   return (float)(2. * M_PI / (double)box);
 }
 
-// Plain path, one radius: separate filter, three inverse passes, moments, barrier.
+// Plain path, one radius: separate filter, three inverse passes, moments and
+// barrier.  T1 holds the filtered spectrum (S layout) and then the real field,
+// T2 the intermediate (T layout).
 int plain_radius(prc_d2z_ctx* c, int r, bool barrier) {
-  const int N = c->N, Nh = c->Nh;
+  const int N = c->N;
   hipStream_t s = c->stream;
   const float dkR = host_dk(c) * c->rc[r].R;
   const float invN3 = (float)(1.0 / ((double)N * N * N));
+  float2* T1 = c->d_T;
+  float2* T2 = c->d_T + c->nspec;
+  const TileMap xm = tile_map(N, Axis::kXStoT), ym = tile_map(N, Axis::kYInPlace);
   PRC_CHECK(c, timed(c, "filter", [&] {
-              return launch_filter(c->d_S, c->d_D0, c->d_DN, c->d_T, N, dkR, invN3, s);
+              return launch_filter(N, c->d_S, c->d_D0, c->d_DN, T1, dkR, invN3, s);
             }),
             "filter");
-  PRC_CHECK(c, timed(c, "inv_x", [&] {
-              return launch_strided(N, +1, c->d_T, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s);
-            }),
+  PRC_CHECK(c, timed(c, "inv_x", [&] { return launch_tile(N, +1, T1, T2, xm, c->d_tw_full, s); }),
             "inverse x pass");
-  PRC_CHECK(c, timed(c, "inv_y", [&] {
-              return launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s);
-            }),
+  PRC_CHECK(c, timed(c, "inv_y", [&] { return launch_tile(N, +1, T2, T2, ym, c->d_tw_full, s); }),
             "inverse y pass");
+  float* dR = reinterpret_cast<float*>(T1);
   PRC_CHECK(c, timed(c, "inv_z_c2r", [&] {
-              return launch_inv_z(Nh, c->d_T, (long long)N * N, c->d_tw_half, c->d_tw_full, s);
+              return launch_inv_z(N, T2, dR, c->d_tw_half, c->d_tw_full, s);
             }),
             "inverse z pass");
   if (!barrier) return PRC_D2Z_OK;
-  const float* dR = reinterpret_cast<const float*>(c->d_T);
   PRC_CHECK(c, timed(c, "moments", [&] {
               return launch_moments(dR, c->ncell, c->d_mom + (long long)r * kMomentBlocks, kMomentBlocks, s);
             }),
@@ -221,21 +220,21 @@ int plain_radius(prc_d2z_ctx* c, int r, bool barrier) {
 }
 
 int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
-  const int N = c->N, Nh = c->Nh;
+  const int N = c->N;
   hipStream_t s = c->stream;
   const float invN3 = (float)(1.0 / ((double)N * N * N));
+  TileMap ym = tile_map(N, Axis::kYInPlace);
+  ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
                                c->d_tw_full, s);
             }),
             "fused x pass");
-  PRC_CHECK(c, timed(c, "inv_y", [&] {
-              return launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, (long long)nb * N, Nh, c->d_tw_full, s);
-            }),
+  PRC_CHECK(c, timed(c, "inv_y", [&] { return launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, s); }),
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
-              return launch_fz(Nh, c->d_T, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, (long long)N * N,
-                               c->d_rc, r0, nb, c->d_part, c->nblk_fz, c->d_tw_half, c->d_tw_full, s);
+              return launch_fz(N, c->d_T, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->d_rc, r0, nb,
+                               c->d_part, c->nblk_fz, c->d_tw_half, c->d_tw_full, s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
@@ -263,13 +262,13 @@ int run_all(prc_d2z_ctx* c) {
       if ((rc = plain_radius(c, r, true))) return rc;
     PRC_CHECK(c, launch_moments_finish(c->d_mom, kMomentBlocks, NS, invN3, c->d_sigma_grid, 0, s), "sigma_grid");
   } else {
-    for (int r0 = 0; r0 < NS; r0 += c->nbuf) {
-      const int nb = std::min(c->nbuf, NS - r0);
+    for (int r0 = 0; r0 < NS; r0 += c->nbatch) {
+      const int nb = std::min(c->nbatch, NS - r0);
       if ((rc = fused_batch(c, r0, nb))) return rc;
     }
     PRC_CHECK(c, launch_sigma_finish(c->d_part, c->nblk_fz, NS, invN3, c->d_sigma_grid, s), "sigma_grid");
   }
-  if (c->timing) hipEventRecord(c->timer.t1, s);
+  if (c->timing) (void)hipEventRecord(c->timer.t1, s);
   c->have_result = true;
   return PRC_D2Z_OK;
 }
@@ -333,10 +332,10 @@ void prc_d2z_destroy(prc_d2z_ctx* c) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipSetDevice(c->p.device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->timer.destroy();
-  for (void* q : c->allocs) hipFree(q);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (void* q : c->allocs) (void)hipFree(q);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipSetDevice(dev);
   delete c;
 }
@@ -389,9 +388,12 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   c->Nh = p->N / 2;
   c->ncell = (long long)p->N * p->N * p->N;
   c->nspec = (long long)p->N * p->N * c->Nh;
-  c->nbuf = (p->mode == PRC_D2Z_PLAIN_FP32) ? 1
-                                            : std::min(p->Nscales, p->radii_per_launch ? p->radii_per_launch
-                                                                                       : kDefaultRadiiPerLaunch);
+  // radii per fused batch; the plain path uses two buffers (filtered spectrum
+  // / real field, and the intermediate), and at least two are always held
+  c->nbatch = (p->mode == PRC_D2Z_PLAIN_FP32)
+                  ? 1
+                  : std::min(p->Nscales, p->radii_per_launch ? p->radii_per_launch : kDefaultRadiiPerLaunch);
+  c->nbuf = std::max(2, c->nbatch);
   auto bail = [&](int code, const std::string& m) {
     g_create_error = m;
     prc_d2z_destroy(c);
@@ -400,7 +402,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   // memory budget check before allocating
   const int NS = p->Nscales;
-  c->nblk_fz = fz_blocks(c->Nh, (long long)p->N * p->N);
+  c->nblk_fz = fz_blocks(p->N);
   size_t need = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) +
                 sizeof(float2) * c->nspec * (1 + c->nbuf) + sizeof(float2) * 2 * (size_t)p->N * p->N;
   size_t freeb = 0, totalb = 0;
@@ -509,12 +511,11 @@ int prc_d2z_generate_delta(prc_d2z_ctx* c, uint64_t seed, int white_noise_only) 
       return rc;
     }
     PRC_CHECK(c, launch_grf_norm(sp, c->d_grfA, s), "field normalisation");
-    PRC_CHECK(c, launch_colour(c->d_S, c->d_D0, c->d_DN, N, sp, c->d_grfA, s), "colouring");
-    PRC_CHECK(c, launch_strided(N, +1, c->d_S, (long long)N * Nh, Nh, N, Nh, c->d_tw_full, s), "field x pass");
-    PRC_CHECK(c, launch_strided(N, +1, c->d_S, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, s), "field y pass");
-    PRC_CHECK(c, launch_inv_z(Nh, c->d_S, (long long)N * N, c->d_tw_half, c->d_tw_full, s), "field z pass");
-    PRC_CHECK(c, hipMemcpyAsync(c->d_delta, c->d_S, sizeof(float) * c->ncell, hipMemcpyDeviceToDevice, s),
-              "field copy");
+    PRC_CHECK(c, launch_colour(N, c->d_S, c->d_D0, c->d_DN, sp, c->d_grfA, s), "colouring");
+    float2* T0 = c->d_T;
+    PRC_CHECK(c, launch_tile(N, +1, c->d_S, T0, tile_map(N, Axis::kXStoT), c->d_tw_full, s), "field x pass");
+    PRC_CHECK(c, launch_tile(N, +1, T0, T0, tile_map(N, Axis::kYInPlace), c->d_tw_full, s), "field y pass");
+    PRC_CHECK(c, launch_inv_z(N, T0, c->d_delta, c->d_tw_half, c->d_tw_full, s), "field z pass");
   }
   PRC_CHECK(c, hipStreamSynchronize(s), "field generation");
   c->timing = was_timing;
@@ -620,18 +621,21 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
     if (c->p.mode == PRC_D2Z_PLAIN_FP32) {
       rc = plain_radius(c, r, false);
     } else {
-      const int N = c->N, Nh = c->Nh;
+      const int N = c->N;
       const float invN3 = (float)(1.0 / ((double)N * N * N));
-      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
-                               c->d_tw_full, c->stream);
-      if (e == hipSuccess) e = launch_strided(N, +1, c->d_T, Nh, (long long)N * Nh, N, Nh, c->d_tw_full, c->stream);
-      if (e == hipSuccess) e = launch_inv_z(Nh, c->d_T, (long long)N * N, c->d_tw_half, c->d_tw_full, c->stream);
+      float2* T0 = c->d_T;
+      float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
+      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, T0, host_dk(c), invN3, c->d_rc, r, 1, c->d_tw_full,
+                               c->stream);
+      if (e == hipSuccess) e = launch_tile(N, +1, T0, T0, tile_map(N, Axis::kYInPlace), c->d_tw_full, c->stream);
+      if (e == hipSuccess) e = launch_inv_z(N, T0, real, c->d_tw_half, c->d_tw_full, c->stream);
       if (e != hipSuccess) rc = c->hip_fail(e, "smoothed field");
     }
   }
   c->timing = was_timing;
   if (rc) return rc;
-  return copy_field_out(c, reinterpret_cast<const float*>(c->d_T), out, PRC_D2Z_DENSE);
+  const float* real = reinterpret_cast<const float*>(c->p.mode == PRC_D2Z_PLAIN_FP32 ? c->d_T : c->d_T + c->nspec);
+  return copy_field_out(c, real, out, PRC_D2Z_DENSE);
 }
 
 int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {

@@ -211,3 +211,53 @@ def test_radix3_n192(mode):
     print(m)
     assert m["max_abs"] <= TOL_Z
     assert m["xhii_max"] <= 1e-3
+
+
+def test_field_generator_matches_oracle():
+    import grf
+    N, box, seed = 64, 100.0, 7
+    lp = lib_params(N, nscales=3)
+    lp.BoxSize = box
+    announce("field generator N=64")
+    with D2Z(lp) as d:
+        d.generate_delta(seed, white_noise_only=True)
+        wn = d.delta()
+        d.generate_delta(seed)
+        g = d.delta().astype(np.float64)
+    ref_wn = grf.white_noise(N, seed)
+    # same Philox words; Box-Muller in double on both sides, rounded to float
+    assert np.max(_ulps(wn, ref_wn)) <= 1
+    assert np.mean(wn == ref_wn) > 0.999
+    ref = grf.grf(N, box, seed).astype(np.float64)
+    assert np.max(np.abs(g - ref)) <= 2e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32])
+def test_c2_size_512(mode):
+    """512^3 / 1000 Mpc/h (the C2 box) on two of its 64 radii: zreion over the
+    subset against the FP64 oracle, plus size-independent properties."""
+    N, box = 512, 1000.0
+    from prc_d2z import D2ZParams
+    lp = D2ZParams(N=N, BoxSize=box, Nscales=64, Rmax=80.0, zeta=40.0, M_min=1.4706e8, mode=mode)
+    announce(f"512^3 mode={mode}")
+    with D2Z(lp) as d:
+        d.generate_delta(1)
+        delta = d.delta()
+        d.run()
+        z = d.zreion()
+        tab = d.radius_table()
+        fields = {r: d.smoothed_field(r) for r in (0, 40)}
+    assert np.all(z >= 0) and np.isfinite(z).all()
+    assert np.all(np.diff(tab[:, 1]) < 0)
+    op = oracle_params(lp)
+    sc = oracle.setup_scalars(op)
+    dhat = oracle.forward(delta, sc.DInit)
+    kmag = oracle.kmag_grid(N, sc.box_mpc)
+    for r, g in fields.items():
+        ref = oracle.smooth(dhat, kmag, sc.R[r])
+        err = np.max(np.abs(g - ref)) / np.sqrt(np.mean(ref ** 2))
+        print(f"512^3 r={r} rel max err {err:.2e}")
+        assert err <= 5e-5
+        # sigma_grid of the run equals the rms of the smoothed field
+        assert tab[r, 2] == pytest.approx(np.sqrt(np.mean(ref ** 2)), rel=1e-5)

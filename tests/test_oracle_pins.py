@@ -1,0 +1,332 @@
+"""Pins of the oracle (oracle/d2z_fp64.py, oracle/grf.py, oracle/d2z_cpu.c).
+
+The reference ships no golden vectors for this path, so each oracle function
+is checked against something independent of its own formula: a closed form, a
+limit, a physical identity, a different algorithm or library, or brute force.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, optimize, special
+
+import d2z_cpu
+import d2z_fp64 as o
+import grf
+
+P = o.D2ZParams()
+
+
+# ------------------------------------------------------------------ FFT use
+def test_forward_and_smooth_of_a_plane_wave():
+    """A single plane wave is an eigenfunction of the filter: delta_R = W(|k|R) delta,
+    and r2c puts N^3/2 at the wave's two modes (numpy pocketfft, unnormalised)."""
+    N, box = 16, 20.0
+    x = np.arange(N)
+    kx, ky = 3, N - 2   # one positive, one negative frequency index
+    d = np.cos(2 * np.pi * (kx * x[:, None, None] + ky * x[None, :, None]) / N) * np.ones((1, 1, N))
+    dhat = o.forward(d, 1.0)
+    assert abs(dhat[kx, ky, 0] - N ** 3 / 2) < 1e-9
+    mask = np.ones(dhat.shape, bool)
+    mask[kx, ky, 0] = mask[N - kx, N - ky, 0] = False
+    assert np.max(np.abs(dhat[mask])) < 1e-9
+    kmag = o.kmag_grid(N, box)
+    R = 2.3
+    k = 2 * np.pi / box * math.hypot(kx, N - ky)
+    dR = o.smooth(dhat, kmag, R)
+    np.testing.assert_allclose(dR, o.tophat_W(k * R) * d, atol=1e-12)
+
+
+def test_kgrid_maps_high_indices_to_negative_frequencies():
+    N, box = 16, 10.0
+    k = o.kmag_grid(N, box)
+    dk = 2 * np.pi / box
+    assert k[N - 1, 0, 0] == pytest.approx(dk)
+    assert k[N // 2, 0, 0] == pytest.approx(N / 2 * dk)   # Nyquist -> -k_N, |k| = N/2 dk
+    assert k[0, 0, N // 2] == pytest.approx(N / 2 * dk)
+
+
+def test_mean_mode_is_zeroed():
+    N = 16
+    d = np.full((N, N, N), 0.7)
+    dR = o.smooth(o.forward(d, 1.0), o.kmag_grid(N, 10.0), 1.0)
+    assert np.max(np.abs(dR)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 48, 64, 96])
+def test_cpu_restatement_fft_matches_brute_force_dft(n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    j = np.arange(n)
+    F = np.exp(-2j * np.pi * np.outer(j, j) / n)
+    ref_f = F @ x.astype(np.complex128)
+    ref_b = F.conj() @ x.astype(np.complex128)
+    assert np.max(np.abs(d2z_cpu.fft1d(x, -1) - ref_f)) < 1e-5 * np.max(np.abs(ref_f))
+    assert np.max(np.abs(d2z_cpu.fft1d(x, +1) - ref_b)) < 1e-5 * np.max(np.abs(ref_b))
+
+
+def test_cpu_restatement_r2c_matches_numpy():
+    N = 16
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((N, N, N)).astype(np.float32)
+    h = d2z_cpu.CpuD2Z(N, 10.0, 2)
+    h.forward(d, 1.0)
+    spec = h.spectrum()
+    h.close()
+    ref = np.fft.rfftn(d.astype(np.float64))
+    assert np.max(np.abs(spec - ref)) < 2e-5 * np.max(np.abs(ref))
+
+
+# -------------------------------------------------------------- top-hat W
+def test_tophat_closed_form_values():
+    assert o.tophat_W(np.array([0.0]))[0] == 1.0
+    assert o.tophat_W(np.array([np.pi]))[0] == pytest.approx(3 / np.pi ** 2, rel=1e-14)
+    # first zero: tan x = x
+    x0 = optimize.brentq(lambda x: math.tan(x) - x, 4.4, 4.6)
+    assert abs(o.tophat_W(np.array([x0]))[0]) < 1e-14
+    # large x: W ~ -3 cos x / x^2
+    x = np.array([1e4 + 0.3])
+    assert o.tophat_W(x)[0] == pytest.approx(-3 * np.cos(x[0]) / x[0] ** 2, rel=1e-3)
+
+
+def test_tophat_series_meets_closed_form():
+    x = np.linspace(0.45, 0.55, 11)
+    closed = 3 * (np.sin(x) - x * np.cos(x)) / x ** 3
+    np.testing.assert_allclose(o.tophat_W(x), closed, rtol=2e-12)
+    xs = np.array([1e-4])
+    assert o.tophat_W(xs)[0] == pytest.approx(1 - xs[0] ** 2 / 10, rel=1e-15)
+
+
+# ----------------------------------------------------------------- growth
+def _growth_exact(z, om, ol, w):
+    """Linear growth of flat wCDM from the ODE in ln a, normalised to D(z=0)=1."""
+    def E2(a):
+        return om * a ** -3 + ol * a ** (-3 * (1 + w))
+
+    def rhs(lna, y):
+        a = math.exp(lna)
+        D, dD = y
+        dlnE = 0.5 * (-3 * om * a ** -3 - 3 * (1 + w) * ol * a ** (-3 * (1 + w))) / E2(a)
+        omz = om * a ** -3 / E2(a)
+        return [dD, -(2 + dlnE) * dD + 1.5 * omz * D]
+
+    a0 = 1e-3
+    sol = integrate.solve_ivp(rhs, [math.log(a0), 0.0], [a0, a0], rtol=1e-10, atol=1e-14,
+                              dense_output=True)
+    Dz = sol.sol(math.log(1 / (1 + z)))[0]
+    return Dz / sol.sol(0.0)[0]
+
+
+def test_growth_einstein_de_sitter_exact():
+    for z in (0.0, 1.0, 7.0, 20.0):
+        assert o.growth(z, 1.0, 0.0, -1.0) == pytest.approx(1 / (1 + z), rel=2e-7)
+
+
+def test_growth_fit_tracks_the_growth_equation():
+    for z in (0.0, 0.5, 2.0, 6.0, 20.0):
+        fit = float(o.growth(z, 0.31, 0.69, -1.0))
+        assert fit == pytest.approx(_growth_exact(z, 0.31, 0.69, -1.0), rel=0.005)
+
+
+def test_growth_with_w_matches_where_matter_dominates():
+    # cosmology.C:188 keeps "+ omegal" (no (1+z)^(3w) factor) in the Omega_m(z)
+    # denominator, so for w != -1 the reference's D(z) departs from the growth
+    # equation by several per cent at z < 2; the oracle reproduces that.  Where
+    # matter dominates both agree.
+    for z in (6.0, 10.0, 20.0):
+        fit = float(o.growth(z, 0.31, 0.69, -0.8))
+        assert fit == pytest.approx(_growth_exact(z, 0.31, 0.69, -0.8), rel=0.01)
+    assert abs(float(o.growth(0.5, 0.31, 0.69, -0.8)) / _growth_exact(0.5, 0.31, 0.69, -0.8) - 1) > 0.03
+
+
+# ---------------------------------------------------------------- transeh
+def test_transeh_limits_and_baryon_suppression():
+    eh = o._EH(0.31, 0.049, 0.68, 0.965, 1.0)
+    assert eh.transeh(1e-7) == pytest.approx(1.0, abs=1e-5)
+    # EH98 eq. 26 sound horizon for omega_m h^2 = 0.1432, omega_b h^2 = 0.02237: ~147 Mpc
+    om, ob = 0.1432, 0.02237
+    s = 44.5 * math.log(9.83 / om) / math.sqrt(1 + 10 * ob ** 0.75)
+    assert s == pytest.approx(147.1, rel=0.03)
+    eh0 = o._EH(0.31, 1e-9, 0.68, 0.965, 1.0)
+    assert eh.transeh(0.3) < eh0.transeh(0.3)          # baryons suppress small scales
+    # with no baryons T depends on k/h only through k/(Omega_m h^2)
+    a = o._EH(0.25, 1e-12, 0.8, 1, 1)
+    b = o._EH(0.4, 1e-12, 0.5, 1, 1)
+    for kh in (0.01, 0.1, 1.0):          # same k/h in both
+        assert a.transeh(kh * 0.8) == pytest.approx(b.transeh(kh * 0.5), rel=1e-6)
+
+
+def test_sigma_integral_in_k_matches_ln_k():
+    """sigma^2 = pnorm^2 int k^(2+ns) T^2 W^2 dk (the ln k Jacobian)."""
+    eh = o._EH(0.31, 0.049, 0.68, 0.965, 1.0)
+    R = 5.0
+    ln_form = eh.integral(R)
+
+    def f(k):
+        return k ** (2 + 0.965) * eh.transeh(k) ** 2 * o.tophat_W(np.array([k * R]))[0] ** 2
+
+    pts = np.logspace(-5, 5, 41) / R
+    k_form = sum(integrate.quad(f, a, b, epsrel=1e-10, limit=400)[0] for a, b in zip(pts[:-1], pts[1:]))
+    assert ln_form == pytest.approx(k_form, rel=1e-6)
+
+
+def test_sigma8_identity():
+    pn = o.normalize(P.Omegam, P.Omegab, P.h, P.ns, P.sigma8)
+    R8 = 8.0 / P.h
+    M8 = 4 * math.pi / 3 * 2.775e11 * P.Omegam * P.h ** 2 * R8 ** 3
+    s8 = math.sqrt(o.cdmehint(P.Omegam, P.Omegab, P.h, P.ns, pn, M8))
+    assert s8 == pytest.approx(P.sigma8, rel=2e-6)
+
+
+def test_sigma_decreases_with_radius_and_stays_below_sigma_min():
+    sc = o.setup_scalars(o.D2ZParams(N=64, BoxSize=100.0, Nscales=10, Rmax=25.0))
+    assert np.all(np.diff(sc.sigma) < 0)
+    assert np.all(sc.sigma < sc.sigma_min)
+
+
+# -------------------------------------------------------------- erfc^-1
+@pytest.mark.parametrize("zeta", [10.0, 40.0, 100.0, 3.0])
+def test_inverse_erfc_is_the_lower_end_of_a_tight_bracket(zeta):
+    x = np.float32(1.0 / np.float32(zeta))
+    root = special.erfcinv(float(x))
+    e = float(o.inverse_erfc(1.0 / zeta))
+    assert root - 1.1e-4 * root <= e <= root + 1e-6
+    assert math.erfc(float(np.float32(e))) >= float(x) * (1 - 1e-6)
+    assert root == pytest.approx(optimize.brentq(lambda t: math.erfc(t) - float(x), 0, 100, xtol=1e-15),
+                                 abs=1e-12)
+
+
+# ---------------------------------------------------------------- barrier
+def test_barrier_solves_the_collapse_condition():
+    """z from the affine closed form satisfies zeta f_coll(delta_R, z) = 1 with
+    f_coll = erfc((deltac/D(z) - delta_R)/sqrt(2(smin^2 - s^2))) and matter-era
+    D(z) = D(20) 21/(1+z)."""
+    p = o.D2ZParams(N=64, BoxSize=100.0, Nscales=6, Rmax=25.0, zeta=40.0)
+    sc = o.setup_scalars(p)
+    rng = np.random.default_rng(2)
+    for r in range(p.Nscales):
+        dR = rng.normal(0, 2, 20)
+        z = o.barrier(dR, sc.A, sc.B[r])
+        D = float(sc.DHigh) * 21 / (1 + z)
+        s = math.sqrt(2 * (float(sc.sigma_min) ** 2 - float(sc.sigma[r]) ** 2))
+        lhs = p.zeta * special.erfc((1.686 / D - dR) / s)
+        np.testing.assert_allclose(lhs, 1.0, rtol=2e-3)
+
+
+def test_barrier_known_answer_at_zero_density():
+    sc = o.setup_scalars(o.D2ZParams(N=32, BoxSize=50.0, Nscales=3, Rmax=10.0))
+    z0 = o.barrier(np.zeros(1), sc.A, sc.B[0])[0]
+    assert z0 == pytest.approx(float(sc.A) * sc.B[0] - 1, rel=1e-15)
+
+
+# -------------------------------------------------------- running maximum
+def test_running_max_is_zero_floored_max_over_radii():
+    N = 16
+    p = o.D2ZParams(N=N, BoxSize=25.0, Nscales=5, Rmax=6.0, zeta=40.0)
+    sc = o.setup_scalars(p)
+    d = grf.grf(N, 25.0, 3)
+    z, _, _ = o.delta2zreion(d, p, sc, return_fp64=True)
+    dhat = o.forward(d, sc.DInit)
+    k = o.kmag_grid(N, sc.box_mpc)
+    zs = [o.barrier(o.smooth(dhat, k, sc.R[r]), sc.A, sc.B[r]) for r in range(p.Nscales)]
+    np.testing.assert_array_equal(z, np.maximum(0.0, np.max(zs, axis=0)))
+
+
+# ---------------------------------------------------------------- history
+def test_history_counts_brute_force():
+    rng = np.random.default_rng(4)
+    z = np.concatenate([rng.uniform(-1, 55, 3000), [0.0, 49.9, 49.9004, 0.0105, 0.009]]).astype(np.float32)
+    dz = float(o.history_dz())
+    ref = np.zeros(o.NHISTORY, np.int64)
+    for v in z:
+        q = (float(v) - 49.9) / dz
+        b = int(q)  # C truncation toward zero
+        if 0 <= b < o.NHISTORY:
+            ref[b] += 1
+    np.testing.assert_array_equal(o.history_counts(z), ref)
+
+
+def test_tau_of_instant_reionization_matches_quadrature():
+    zr = 8.0
+    N = 8
+    z = np.full(N ** 3, zr, np.float32)
+    counts = o.history_counts(z)
+    x = o.history_float(counts, N)
+    za = o.history_z()
+    t = float(o.tau(za, x, 0.31, 0.049, 0.68))
+    h = 0.68
+    ne0 = 0.049 * h * h * 1.88e-29 / 1.67e-24
+    dtau0 = 6.65e-25 * ne0 * 3e10 / (100.0 / 3.086e19 * h)
+
+    def f(zz):
+        nhe = 2 if zz < 3 else 1
+        return (1 + zz) ** 2 / math.sqrt(0.31 * (1 + zz) ** 3 + 1 - 0.31) * (1 - 0.25 + nhe * 0.25 / 4)
+
+    ref = dtau0 * (integrate.quad(f, 0.01, 3)[0] + integrate.quad(f, 3, zr)[0])
+    assert t == pytest.approx(ref, rel=2e-3)
+
+
+# --------------------------------------------------------------------- GRF
+def test_philox_known_answer_vectors():
+    # Random123 known-answer vectors for philox4x32-10
+    cases = [
+        ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in cases:
+        got = grf.philox4x32_10(*[np.array([c]) for c in ctr], *key)
+        assert tuple(int(g[0]) for g in got) == want
+
+
+def test_normals_are_standard():
+    g = grf.cell_normals(7, 0, 200000).astype(np.float64)
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1) < 0.01
+    assert abs(np.mean(g ** 4) - 3) < 0.06
+
+
+def test_grf_spectrum_and_normalisation():
+    # sigma(8 Mpc/h) of the spectrum from an independent k-space quadrature
+    A = grf.amplitude()
+    gam = grf.bbks_gamma(0.31, 0.049, 0.68)
+
+    def f(k):
+        return A * k ** 2 * k ** 0.965 * grf.t_bbks(k, gam) ** 2 * grf._w_tophat(8 * k) ** 2 / (2 * np.pi ** 2)
+
+    pts = np.logspace(-6, 4, 51)
+    s2 = sum(integrate.quad(f, a, b, limit=200)[0] for a, b in zip(pts[:-1], pts[1:]))
+    assert math.sqrt(s2) == pytest.approx(0.81, rel=1e-6)
+    assert float(grf.t_bbks(1e-8, gam)) == pytest.approx(1.0, abs=1e-6)
+    # measured power of a 64^3 field against P(k) in shells
+    N, box = 64, 100.0
+    d = grf.grf(N, box, 11).astype(np.float64)
+    pk = np.abs(np.fft.rfftn(d)) ** 2 * (box / N) ** 3 / N ** 3
+    kk = np.fft.fftfreq(N, 1.0 / N)
+    kz = np.arange(N // 2 + 1)
+    kmag = np.sqrt(kk[:, None, None] ** 2 + kk[None, :, None] ** 2 + kz[None, None, :] ** 2) * 2 * np.pi / box
+    for lo, hi in [(0.2, 0.4), (0.4, 0.8), (0.8, 1.6)]:
+        sel = (kmag >= lo) & (kmag < hi)
+        ratio = np.mean(pk[sel]) / np.mean(grf.power(kmag[sel]))
+        assert ratio == pytest.approx(1.0, rel=0.08)
+
+
+# --------------------------------------------------- two restatements agree
+@pytest.mark.parametrize("N", [16, 32])
+def test_c_restatement_agrees_with_fp64_oracle(N):
+    box = 1.5625 * N
+    p = o.D2ZParams(N=N, BoxSize=box, Nscales=6, Rmax=box / 4, zeta=40.0)
+    sc = o.setup_scalars(p)
+    d = grf.grf(N, box, 1)
+    zc, sgc = d2z_cpu.run(d, p, sc, nthreads=2)
+    zr, sgr, _ = o.delta2zreion(d, p, sc)
+    assert np.max(np.abs(zc.astype(np.float64) - zr)) < 1e-4
+    np.testing.assert_allclose(sgc, sgr, rtol=1e-5)
+
+
+def test_radius_schedule_endpoints_and_spacing():
+    p = o.D2ZParams(N=128, BoxSize=200.0, Nscales=32, Rmax=50.0)
+    R = o.radius_schedule(p).astype(np.float64)
+    assert R[0] == pytest.approx(200.0 / 0.68 / 128, rel=1e-6)
+    assert R[-1] == pytest.approx(50.0 / 0.68, rel=1e-5)
+    np.testing.assert_allclose(np.diff(np.log(R)), np.log(R[-1] / R[0]) / 31, rtol=1e-4)
