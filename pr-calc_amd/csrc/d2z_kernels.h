@@ -56,11 +56,17 @@ hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const Tile
 hipError_t launch_unpack(int N, const float2* S, float2* D0, float2* DN, hipStream_t s);
 hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
                          float invN3, hipStream_t s);
-hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
-                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
+hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const float2* twh, const float2* twf,
+                          hipStream_t s);
+// in-place tiled pass over FP16 intermediates (mixed path), output scaled
+hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, const float2* tw, hipStream_t s);
+// fused passes; Tbuf holds float2 (half = false) or __half2 (half = true)
+hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
+                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
 int fz_blocks(int N);
-hipError_t launch_fz(int N, const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                     double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s);
+hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
+                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                     hipStream_t s);
 // element-wise and set-up kernels (d2z_kernels.hip)
 hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long total, const RadiusConst* rc, int r,
                           hipStream_t s);

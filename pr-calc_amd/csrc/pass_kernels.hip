@@ -14,6 +14,7 @@
 //   forward  z r2c (rows -> T layout), y (in place), x (T layout -> S)
 //   inverse  x with the filter fused (S -> T_r), y (in place), z c2r with the
 //            barrier and running maximum fused (T_r -> zreion)
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -23,6 +24,14 @@
 #include "fft_core.h"
 
 namespace prc {
+
+// ------------------------------------------------------ intermediate storage
+// Intermediates are float2 (FP32 path) or __half2 (mixed path: FP32
+// arithmetic, FP16 storage, round to nearest even).
+__device__ __forceinline__ float2 ld(const float2* p, long long i) { return p[i]; }
+__device__ __forceinline__ float2 ld(const __half2* p, long long i) { return __half22float2(p[i]); }
+__device__ __forceinline__ void st(float2* p, long long i, float2 v) { p[i] = v; }
+__device__ __forceinline__ void st(__half2* p, long long i, float2 v) { p[i] = __float22half2_rn(v); }
 
 // ---------------------------------------------------- real-transform glue
 // Forward untangling of Z = FFT_L(x[2n] + i x[2n+1]) into the packed half
@@ -107,31 +116,31 @@ __global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, flo
 // Tile t -> (o, cb) = (t mod nOuter, t / nOuter); element (a, c) is read at
 // cb*sCB + o*in_sO + a*in_sA + c and written at cb*sCB + o*out_sO + a*out_sA + c
 // (in place when in == out with equal strides).
-template <class P, int C, int DIR>
+template <class P, int C, int DIR, class TI, class TO>
 __global__ void __launch_bounds__(P::T* C)
-    k_tile(const float2* in, float2* out, long long in_sA, long long in_sO, long long out_sA, long long out_sO,
-           long long sCB, int nOuter, const float2* __restrict__ tw) {
+    k_tile(const TI* in, TO* out, long long in_sA, long long in_sO, long long out_sA, long long out_sO,
+           long long sCB, int nOuter, float oscale, const float2* __restrict__ tw) {
   constexpr int L = P::L, T = P::T, E = P::E;
   __shared__ float2 smem[L * C];
   const int tid = threadIdx.x;
   const int c = tid % C, i = tid / C;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   const long long o = tile % nOuter, cb = tile / nOuter;
-  const float2* ib = in + cb * sCB + o * in_sO + c;
-  float2* ob = out + cb * sCB + o * out_sO + c;
+  const TI* ib = in + cb * sCB + o * in_sO + c;
+  TO* ob = out + cb * sCB + o * out_sO + c;
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = ib[(long long)(i + e * T) * in_sA];
+  for (int e = 0; e < E; ++e) v[e] = ld(ib, (long long)(i + e * T) * in_sA);
   pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
 #pragma unroll
-  for (int e = 0; e < E; ++e) ob[(long long)(i + e * T) * out_sA] = v[e];
+  for (int e = 0; e < E; ++e) st(ob, (long long)(i + e * T) * out_sA, cscale(v[e], oscale));
 }
 
 // ---------------------------------------------------- inverse z pass (c2r)
 // Rows (x, y) gathered from the T layout -> dense real rows.
-template <class P, int C>
-__global__ void __launch_bounds__(256) k_inv_z(const float2* __restrict__ in, float* __restrict__ out,
-                                               long long nrows, const float2* __restrict__ tw_half,
+template <class P, int C, class TS>
+__global__ void __launch_bounds__(256) k_inv_z(const TS* __restrict__ in, float* __restrict__ out, long long nrows,
+                                               float iscale, const float2* __restrict__ tw_half,
                                                const float2* __restrict__ tw_full) {
   constexpr int L = P::L, T = P::T, E = P::E;
   constexpr int ROWS = 256 / T;
@@ -145,7 +154,7 @@ __global__ void __launch_bounds__(256) k_inv_z(const float2* __restrict__ in, fl
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = i + e * T;
-    v[e] = in[((long long)(k / C) * nrows + r) * C + (k % C)];
+    v[e] = ld(in, ((long long)(k / C) * nrows + r) * C + (k % C));
   }
   mirror_lanes<P>(v, m, i);
   c2r_pre<P>(v, m, i, tw_full);
@@ -153,7 +162,7 @@ __global__ void __launch_bounds__(256) k_inv_z(const float2* __restrict__ in, fl
   if (valid) {
     float2* dst = reinterpret_cast<float2*>(out + row * (2 * L));
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[i + e * T] = v[e];
+    for (int e = 0; e < E; ++e) dst[i + e * T] = cscale(v[e], iscale);
   }
 }
 
@@ -206,16 +215,27 @@ __global__ void k_filter(const float2* __restrict__ S, const float2* __restrict_
 // spectrum values are read once into registers and reused for nb radii: each
 // radius scales by W(|k| R_r)/N^3, transforms along x and writes its own
 // intermediate T_r = Tbuf + r_local * N*N*N/2 (T layout).
-template <class P, int C, int CF>
+//
+// W depends on kx only through |fx| (fx = kx or kx - N), so per radius the
+// block first tabulates W for its CF columns and the N/2 + 1 values of |fx|
+// in LDS (half the filter evaluations of a per-element loop), plus the
+// kz = N/2 column for the packed slot.
+template <class P, int C, int CF, class TS>
 __global__ void __launch_bounds__(P::T* CF)
     k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
-         float2* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
+         TS* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
          const float2* __restrict__ tw) {
   constexpr int N = P::L, T = P::T, E = P::E;
   constexpr int NSUB = C / CF;
+  constexpr int NF = N / 2 + 1;           // distinct |fx|
+  constexpr int NT = P::T * CF;           // threads
   constexpr long long Nh = N / 2;
   constexpr long long plane = (long long)N * N * Nh;
   __shared__ float2 smem[N * CF];
+  // W table [column][|fx|] (row CF: kz = N/2 for the packed slot); it aliases
+  // the transform's exchange buffer, which is idle between radii
+  static_assert((CF + 1) * NF <= 2 * N * CF, "W table must fit in the exchange buffer");
+  float* wtab = reinterpret_cast<float*>(smem);
   const int tid = threadIdx.x;
   const int cl = tid % CF, i = tid / CF;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -225,6 +245,7 @@ __global__ void __launch_bounds__(P::T* CF)
   const int c = sub * CF + cl;
   const int kz = kzb * C + c;
   const bool packed = (kz == 0);
+  const bool has_packed = (kzb == 0 && sub == 0);  // block-uniform
   const float2* sp = S + ((long long)kzb * N + ky) * N * C + c;  // element kx at sp[kx*C]
   float2 m[E];
 #pragma unroll
@@ -232,30 +253,38 @@ __global__ void __launch_bounds__(P::T* CF)
     const int kx = i + e * T;
     m[e] = packed ? D0[(long long)kx * N + ky] : sp[(long long)kx * C];
   }
-  const int base2 = fsq(ky, N) + kz * kz;
-  float2* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + c;  // element x at tp[x*N*C]
+  const int fy2 = fsq(ky, N);
+  TS* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + c;  // element x at tp[x*N*C]
   const LdsPencil lds{smem + cl, CF};
+  const int ntab = (has_packed ? CF + 1 : CF) * NF;
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
+    // tabulate W(|k| R)/N^3 for this block's columns
+    for (int q = tid; q < ntab; q += NT) {
+      const int col = q / NF, f = q % NF;
+      const int kzq = (col < CF) ? kzb * C + sub * CF + col : N / 2;
+      wtab[q] = filter_coef(f * f + fy2 + kzq * kzq, dkR, invN3);
+    }
+    __syncthreads();
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kx = i + e * T;
-      const int msq = fsq(kx, N) + base2;
+      const int fa = (kx <= N / 2) ? kx : N - kx;
       if (!packed) {
-        v[e] = cscale(m[e], filter_coef(msq, dkR, invN3));
+        v[e] = cscale(m[e], wtab[cl * NF + fa]);
       } else {
-        const float w0 = filter_coef(msq, dkR, invN3);
-        const float wn = filter_coef(msq + (int)(Nh * Nh), dkR, invN3);
+        const float w0 = wtab[cl * NF + fa], wn = wtab[CF * NF + fa];
         const float2 b = DN[(long long)kx * N + ky];
         v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
       }
     }
+    __syncthreads();  // table reads done before the transform reuses the buffer
     pencil_fft<P, +1>(v, i, tw, lds);
-    float2* dst = tp + (long long)rl * plane;
+    TS* dst = tp + (long long)rl * plane;
 #pragma unroll
-    for (int e = 0; e < E; ++e) dst[(long long)(i + e * T) * N * C] = v[e];
+    for (int e = 0; e < E; ++e) st(dst, (long long)(i + e * T) * N * C, v[e]);
   }
 }
 
@@ -272,10 +301,10 @@ struct FxCols {
 // zreion (and rbubble) is read and written once per batch.  Per-radius sums of
 // delta_R^2 (float products, double sums as GetMeanSigma) are kept per wave in
 // LDS and written as one partial per block and radius.
-template <class P, int C, bool RB>
+template <class P, int C, bool RB, class TS>
 __global__ void __launch_bounds__(256)
-    k_fz(const float2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
-         const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial,
+    k_fz(const TS* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
+         const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
          const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
   constexpr int L = P::L, T = P::T, E = P::E;
   constexpr int ROWS = 256 / T;
@@ -303,24 +332,26 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll 1
     for (int q = 0; q < nb; ++q) {
-      const float2* tp = Tbuf + (long long)q * plane;
+      const TS* tp = Tbuf + (long long)q * plane;
       float2 v[E], m[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int k = i + e * T;
-        v[e] = tp[((long long)(k / C) * nrows + r) * C + (k % C)];
+        v[e] = ld(tp, ((long long)(k / C) * nrows + r) * C + (k % C));
       }
       mirror_lanes<P>(v, m, i);
       c2r_pre<P>(v, m, i, tw_full);
       pencil_fft<P, +1>(v, i, tw_half, lds);
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = cscale(v[e], iscale);  // power of two: exact
       const double A = rc[r0 + q].A, B = rc[r0 + q].B;
       const float R = rc[r0 + q].R;
-      double acc = 0.0;
+      float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const float z0 = barrier_z(v[e].x, A, B);
         const float z1 = barrier_z(v[e].y, A, B);
-        acc += (double)(v[e].x * v[e].x) + (double)(v[e].y * v[e].y);
+        acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
         if (z0 > best[e].x) {
           best[e].x = z0;
           if (RB) bR0[e] = R;
@@ -330,8 +361,8 @@ __global__ void __launch_bounds__(256)
           if (RB) bR1[e] = R;
         }
       }
-      acc = wave_sum(valid ? acc : 0.0);
-      if (lane == 0) red[w][q] += acc;
+      const double wsum = wave_sum(valid ? (double)acc : 0.0);
+      if (lane == 0) red[w][q] += wsum;
     }
     if (valid) {
       float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
@@ -373,23 +404,25 @@ static hipError_t fwd_z_impl(const float* in, float2* out, float dinit, const fl
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t inv_z_impl(const float2* in, float* out, const float2* twh, const float2* twf, hipStream_t s) {
+template <int N, class TS>
+static hipError_t inv_z_impl(const TS* in, float* out, float iscale, const float2* twh, const float2* twf,
+                             hipStream_t s) {
   using P = Plan<N / 2>;
   constexpr int ROWS = 256 / P::T;
   const long long nrows = (long long)N * N;
-  hipLaunchKernelGGL((k_inv_z<P, Blk<N>::C>), dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(256), 0, s, in,
-                     out, nrows, twh, twf);
+  hipLaunchKernelGGL((k_inv_z<P, Blk<N>::C, TS>), dim3((unsigned)((nrows + ROWS - 1) / ROWS)), dim3(256), 0, s, in,
+                     out, nrows, iscale, twh, twf);
   return hipGetLastError();
 }
 
-template <int N, int DIR>
-static hipError_t tile_impl(const float2* in, float2* out, const TileMap& mp, const float2* tw, hipStream_t s) {
+template <int N, int DIR, class TI, class TO>
+static hipError_t tile_impl(const TI* in, TO* out, const TileMap& mp, float oscale, const float2* tw,
+                            hipStream_t s) {
   using P = Plan<N>;
   constexpr int C = Blk<N>::C;
   const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
-  hipLaunchKernelGGL((k_tile<P, C, DIR>), dim3((unsigned)ntiles), dim3(P::T * C), 0, s, in, out, mp.in_sA,
-                     mp.in_sO, mp.out_sA, mp.out_sO, mp.sCB, mp.nOuter, tw);
+  hipLaunchKernelGGL((k_tile<P, C, DIR, TI, TO>), dim3((unsigned)ntiles), dim3(P::T * C), 0, s, in, out, mp.in_sA,
+                     mp.in_sO, mp.out_sA, mp.out_sO, mp.sCB, mp.nOuter, oscale, tw);
   return hipGetLastError();
 }
 
@@ -410,14 +443,14 @@ static hipError_t filter_impl(const float2* S, const float2* D0, const float2* D
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
+template <int N, class TS>
+static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, TS* Tbuf, float dk, float invN3,
                           const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
   using P = Plan<N>;
   constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
   const long long nblk = (long long)N * ((N / 2) / C) * (C / CF);
-  hipLaunchKernelGGL((k_fx<P, C, CF>), dim3((unsigned)nblk), dim3(P::T * CF), 0, s, S, D0, DN, Tbuf, dk, invN3, rc,
-                     r0, nb, tw);
+  hipLaunchKernelGGL((k_fx<P, C, CF, TS>), dim3((unsigned)nblk), dim3(P::T * CF), 0, s, S, D0, DN, Tbuf, dk, invN3,
+                     rc, r0, nb, tw);
   return hipGetLastError();
 }
 
@@ -428,18 +461,19 @@ static int fz_grid(long long nrows) {
   return (int)(need < 2048 ? need : 2048);
 }
 
-template <int N>
-static hipError_t fz_impl(const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                          double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s) {
+template <int N, class TS>
+static hipError_t fz_impl(const TS* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                          double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                          hipStream_t s) {
   using P = Plan<N / 2>;
   constexpr int C = Blk<N>::C;
   const long long nrows = (long long)N * N;
   if (rbub)
-    hipLaunchKernelGGL((k_fz<P, C, true>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb, partial,
-                       twh, twf);
+    hipLaunchKernelGGL((k_fz<P, C, true, TS>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
+                       partial, iscale, twh, twf);
   else
-    hipLaunchKernelGGL((k_fz<P, C, false>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
-                       partial, twh, twf);
+    hipLaunchKernelGGL((k_fz<P, C, false, TS>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
+                       partial, iscale, twh, twf);
   return hipGetLastError();
 }
 
@@ -500,7 +534,20 @@ hipError_t launch_inv_z(int N, const float2* in, float* out, const float2* twh, 
   switch (N) {
 #define X(NN) \
   case NN:    \
-    return inv_z_impl<NN>(in, out, twh, twf, s);
+    return inv_z_impl<NN, float2>(in, out, 1.0f, twh, twf, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const float2* twh, const float2* twf,
+                          hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return inv_z_impl<NN, __half2>(static_cast<const __half2*>(in), out, iscale, twh, twf, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
@@ -511,9 +558,23 @@ hipError_t launch_inv_z(int N, const float2* in, float* out, const float2* twh, 
 hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const TileMap& mp, const float2* tw,
                        hipStream_t s) {
   switch (N) {
-#define X(NN)                                                       \
-  case NN:                                                          \
-    return dir > 0 ? tile_impl<NN, +1>(in, out, mp, tw, s) : tile_impl<NN, -1>(in, out, mp, tw, s);
+#define X(NN)                                                                          \
+  case NN:                                                                             \
+    return dir > 0 ? tile_impl<NN, +1, float2, float2>(in, out, mp, 1.0f, tw, s)      \
+                   : tile_impl<NN, -1, float2, float2>(in, out, mp, 1.0f, tw, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
+  __half2* d = static_cast<__half2*>(data);
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return tile_impl<NN, +1, __half2, __half2>(d, d, mp, oscale, tw, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
@@ -546,13 +607,14 @@ hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2*
   }
 }
 
-hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, float2* Tbuf, float dk, float invN3,
-                     const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
+                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
   if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
   switch (N) {
-#define X(NN) \
-  case NN:    \
-    return fx_impl<NN>(S, D0, DN, Tbuf, dk, invN3, rc, r0, nb, tw, s);
+#define X(NN)                                                                                             \
+  case NN:                                                                                                \
+    return half ? fx_impl<NN, __half2>(S, D0, DN, static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw, s) \
+                : fx_impl<NN, float2>(S, D0, DN, static_cast<float2*>(Tbuf), dk, invN3, rc, r0, nb, tw, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
@@ -572,13 +634,17 @@ int fz_blocks(int N) {
   }
 }
 
-hipError_t launch_fz(int N, const float2* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                     double* partial, int nblk, const float2* twh, const float2* twf, hipStream_t s) {
+hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
+                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                     hipStream_t s) {
   if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
   switch (N) {
-#define X(NN) \
-  case NN:    \
-    return fz_impl<NN>(Tbuf, zre, rbub, rc, r0, nb, partial, nblk, twh, twf, s);
+#define X(NN)                                                                                                 \
+  case NN:                                                                                                    \
+    return half ? fz_impl<NN, __half2>(static_cast<const __half2*>(Tbuf), zre, rbub, rc, r0, nb, partial, nblk, \
+                                       iscale, twh, twf, s)                                                   \
+                : fz_impl<NN, float2>(static_cast<const float2*>(Tbuf), zre, rbub, rc, r0, nb, partial, nblk,   \
+                                      iscale, twh, twf, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

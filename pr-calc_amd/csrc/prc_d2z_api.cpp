@@ -6,6 +6,7 @@
 //   main_d2z.C:52-59  ReadDeltaFile destination, delta /= DInit       -> prc_d2z_set_delta + run
 //   main_d2z.C:67     fftwf_execute(plan)                             -> run: forward passes
 //   main_d2z.C:71     delta2zreion()                                  -> run: radius loop
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -87,8 +88,8 @@ struct prc_d2z_ctx {
   float2* d_D0 = nullptr;
   float2* d_DN = nullptr;
   float2* d_T = nullptr;
-  int nbuf = 2;    // intermediate buffers held
-  int nbatch = 1;  // radii per fused batch
+  int nbatch = 1;     // radii per fused batch
+  size_t tbytes = 0;  // bytes of the intermediate buffer region
   float* d_zre = nullptr;
   float* d_rb = nullptr;
   float2* d_tw_full = nullptr;
@@ -219,22 +220,40 @@ int plain_radius(prc_d2z_ctx* c, int r, bool barrier) {
   return PRC_D2Z_OK;
 }
 
+// Scales of the mixed path (powers of two, so rescaling is exact): the fused
+// x pass multiplies by s1 = 2^round(log2 N), the y pass by s2 =
+// 2^-round(log2 sqrt N), the z pass undoes both; intermediates then stay near
+// the magnitude of delta_R, far inside the FP16 range.
+struct MixedScales {
+  float s1, s2, inv;
+};
+MixedScales mixed_scales(int N) {
+  const int e1 = (int)lround(log2((double)N));
+  const int e2 = (int)lround(0.5 * log2((double)N));
+  return {ldexpf(1.0f, e1), ldexpf(1.0f, -e2), ldexpf(1.0f, e2 - e1)};
+}
+
 int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   const int N = c->N;
   hipStream_t s = c->stream;
-  const float invN3 = (float)(1.0 / ((double)N * N * N));
+  const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+  const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
+  const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
   TileMap ym = tile_map(N, Axis::kYInPlace);
   ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
-              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
+              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
                                c->d_tw_full, s);
             }),
             "fused x pass");
-  PRC_CHECK(c, timed(c, "inv_y", [&] { return launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, s); }),
+  PRC_CHECK(c, timed(c, "inv_y", [&] {
+              return half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_tw_full, s)
+                          : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, s);
+            }),
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
-              return launch_fz(N, c->d_T, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->d_rc, r0, nb,
-                               c->d_part, c->nblk_fz, c->d_tw_half, c->d_tw_full, s);
+              return launch_fz(N, c->d_T, half, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->d_rc, r0, nb,
+                               c->d_part, c->nblk_fz, ms.inv, c->d_tw_half, c->d_tw_full, s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
@@ -361,8 +380,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     g_create_error = "parameters out of range (need h, BoxSize, Rmin, Rmax, M_min, Omegam, sigma8 > 0 and zeta > 1)";
     return PRC_D2Z_ERR_BAD_ARG;
   }
-  if (p->mode != PRC_D2Z_PLAIN_FP32 && p->mode != PRC_D2Z_FUSED_FP32) {
-    g_create_error = "mode not available in this build";
+  if (p->mode != PRC_D2Z_PLAIN_FP32 && p->mode != PRC_D2Z_FUSED_FP32 && p->mode != PRC_D2Z_FUSED_MIXED) {
+    g_create_error = "unknown mode";
     return PRC_D2Z_ERR_BAD_ARG;
   }
   if (p->radii_per_launch < 0 || p->radii_per_launch > 8) {
@@ -393,7 +412,6 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   c->nbatch = (p->mode == PRC_D2Z_PLAIN_FP32)
                   ? 1
                   : std::min(p->Nscales, p->radii_per_launch ? p->radii_per_launch : kDefaultRadiiPerLaunch);
-  c->nbuf = std::max(2, c->nbatch);
   auto bail = [&](int code, const std::string& m) {
     g_create_error = m;
     prc_d2z_destroy(c);
@@ -403,8 +421,12 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   // memory budget check before allocating
   const int NS = p->Nscales;
   c->nblk_fz = fz_blocks(p->N);
-  size_t need = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) +
-                sizeof(float2) * c->nspec * (1 + c->nbuf) + sizeof(float2) * 2 * (size_t)p->N * p->N;
+  // intermediates: the fused batch (FP32 or FP16 storage), and at least two
+  // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
+  const size_t tstore = (p->mode == PRC_D2Z_FUSED_MIXED) ? sizeof(__half2) : sizeof(float2);
+  c->tbytes = std::max((size_t)2 * sizeof(float2) * c->nspec, tstore * c->nspec * c->nbatch);
+  size_t need = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) + sizeof(float2) * c->nspec +
+                c->tbytes + sizeof(float2) * 2 * (size_t)p->N * p->N;
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && need + (64u << 20) > freeb)
     return bail(PRC_D2Z_ERR_OOM, "device memory: need " + std::to_string(need >> 20) + " MiB, free " +
@@ -418,7 +440,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_S, sizeof(float2) * c->nspec);
   A_(c->d_D0, sizeof(float2) * (size_t)p->N * p->N);
   A_(c->d_DN, sizeof(float2) * (size_t)p->N * p->N);
-  A_(c->d_T, sizeof(float2) * c->nspec * c->nbuf);
+  A_(c->d_T, c->tbytes);
   A_(c->d_zre, sizeof(float) * c->ncell);
   if (p->RecordBubbleSizes) A_(c->d_rb, sizeof(float) * c->ncell);
   A_(c->d_tw_full, sizeof(float2) * p->N);
@@ -622,13 +644,20 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       rc = plain_radius(c, r, false);
     } else {
       const int N = c->N;
-      const float invN3 = (float)(1.0 / ((double)N * N * N));
-      float2* T0 = c->d_T;
+      const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+      const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
+      const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
+      // the real output goes behind the first intermediate (FP32 sized region)
       float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
-      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, T0, host_dk(c), invN3, c->d_rc, r, 1, c->d_tw_full,
-                               c->stream);
-      if (e == hipSuccess) e = launch_tile(N, +1, T0, T0, tile_map(N, Axis::kYInPlace), c->d_tw_full, c->stream);
-      if (e == hipSuccess) e = launch_inv_z(N, T0, real, c->d_tw_half, c->d_tw_full, c->stream);
+      const TileMap ym = tile_map(N, Axis::kYInPlace);
+      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
+                               c->d_tw_full, c->stream);
+      if (e == hipSuccess)
+        e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_tw_full, c->stream)
+                 : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, c->stream);
+      if (e == hipSuccess)
+        e = half ? launch_inv_z_h(N, c->d_T, real, ms.inv, c->d_tw_half, c->d_tw_full, c->stream)
+                 : launch_inv_z(N, c->d_T, real, c->d_tw_half, c->d_tw_full, c->stream);
       if (e != hipSuccess) rc = c->hip_fail(e, "smoothed field");
     }
   }

@@ -1,0 +1,80 @@
+"""Mixed path (FP32 arithmetic, FP16 storage of the two inter-pass
+intermediates) against the FP64 oracle.
+
+Tolerances (stated for the mixed path): max |dzreion| <= 0.02, the fraction of
+cells with |dzreion| > 0.01 at most 1e-3, x_HII(z) within 1e-3; the per-radius
+delta_R error is reported and bounded by 2e-3 of the field's rms (FP16 has a
+unit roundoff of 4.9e-4 and each value is rounded twice).
+"""
+import numpy as np
+import pytest
+
+import d2z_fp64 as oracle
+from _cases import field, lib_params, oracle_params, zreion_metrics
+from conftest import announce
+from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_mixed_zreion_vs_oracle(N):
+    lp = lib_params(N, nscales=10, mode=FUSED_MIXED)
+    announce(f"mixed zreion N={N}")
+    with D2Z(lp) as d:
+        d.set_delta(field(N))
+        d.run()
+        z = d.zreion()
+        tab = d.radius_table()
+    op = oracle_params(lp)
+    sc = oracle.setup_scalars(op)
+    zr, sg, _ = oracle.delta2zreion(field(N), op, sc)
+    m = zreion_metrics(z, zr)
+    print(m)
+    assert m["max_abs"] <= 0.02
+    assert m["frac_gt_0.01"] <= 1e-3
+    assert m["xhii_max"] <= 1e-3
+    np.testing.assert_allclose(tab[:, 2], sg, rtol=2e-3)
+
+
+def test_mixed_smoothed_field_error_per_radius():
+    N = 128
+    lp = lib_params(N, nscales=8, mode=FUSED_MIXED)
+    op = oracle_params(lp)
+    sc = oracle.setup_scalars(op)
+    dhat = oracle.forward(field(N), sc.DInit)
+    kmag = oracle.kmag_grid(N, sc.box_mpc)
+    with D2Z(lp) as d:
+        d.set_delta(field(N))
+        for r in range(lp.Nscales):
+            announce(f"mixed smoothed field N={N} r={r}")
+            g = d.smoothed_field(r).astype(np.float64)
+            ref = oracle.smooth(dhat, kmag, sc.R[r])
+            rms = np.sqrt(np.mean(ref ** 2))
+            err_rms = np.sqrt(np.mean((g - ref) ** 2)) / rms
+            err_max = np.max(np.abs(g - ref)) / rms
+            print(f"r={r} R={sc.R[r]:.3f} Mpc rel rms err {err_rms:.2e} rel max err {err_max:.2e}")
+            assert err_rms <= 2e-3
+            assert err_max <= 2e-2
+
+
+@pytest.mark.slow
+def test_mixed_against_fp32_at_512():
+    N = 512
+    outs = {}
+    for mode in (FUSED_FP32, FUSED_MIXED):
+        lp = D2ZParams(N=N, BoxSize=1000.0, Nscales=64, Rmax=80.0, zeta=40.0, M_min=1.4706e8, mode=mode)
+        announce(f"512^3 mode={mode}")
+        with D2Z(lp) as d:
+            d.generate_delta(1)
+            d.run()
+            outs[mode] = (d.zreion(), d.history())
+    z32, h32 = outs[FUSED_FP32]
+    z16, h16 = outs[FUSED_MIXED]
+    dz = np.abs(z16.astype(np.float64) - z32)
+    x32 = np.cumsum(h32.astype(np.int64)) / z32.size
+    x16 = np.cumsum(h16.astype(np.int64)) / z16.size
+    print(f"512^3 mixed vs fp32: max|dz| {dz.max():.3e}  frac>0.01 {np.mean(dz > 0.01):.2e} "
+          f"x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
+    assert dz.max() <= 0.02
+    assert np.max(np.abs(x32 - x16)) <= 1e-3
