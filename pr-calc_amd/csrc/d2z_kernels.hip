@@ -96,14 +96,27 @@ __global__ void k_nonfinite(const float* __restrict__ x, long long n, int* __res
 
 // ------------------------------------------------ history histogram (GPU)
 // bin = (int)((z - 49.9)/dzhistory) in double (bubblehistory_d2z.C:17-28);
-// integer counts.
-__global__ void k_history(const float* __restrict__ zre, long long total, unsigned int* __restrict__ counts,
-                          double dz) {
+// integer counts.  Block (chunk, window) counts the bins of its window in an
+// LDS histogram (LDS atomics), then adds the non-zero bins to the global counts.
+constexpr int kHistWindow = 25000;  // bins per window: 4 windows cover NHISTORY
+__global__ void __launch_bounds__(1024) k_history(const float* __restrict__ zre, long long total,
+                                                  unsigned int* __restrict__ counts, double dz) {
+  __shared__ unsigned int h[kHistWindow];
+  const int w = blockIdx.y;
+  const int lo = w * kHistWindow;
+  for (int b = threadIdx.x; b < kHistWindow; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
   for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
        id += (long long)gridDim.x * blockDim.x) {
     const double q = ((double)zre[id] - 49.9) / dz;
-    if (q > -1.0 && q < (double)kNHistory) atomicAdd(&counts[(int)q], 1u);
+    if (q > -1.0 && q < (double)kNHistory) {
+      const int b = (int)q - lo;
+      if (b >= 0 && b < kHistWindow) atomicAdd(&h[b], 1u);
+    }
   }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kHistWindow; b += blockDim.x)
+    if (h[b]) atomicAdd(&counts[lo + b], h[b]);
 }
 
 // =================================================== FP64 scalar set-up
@@ -399,7 +412,10 @@ hipError_t launch_moments_finish(const double2* partial, int nblk, int nrad, dou
 }
 
 hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, hipStream_t s) {
-  hipLaunchKernelGGL(k_history, dim3(grid1(total, 256, 8192)), dim3(256), 0, s, zre, total, counts, dz);
+  static_assert(4 * kHistWindow == kNHistory, "windows must tile the history bins");
+  long long g = (total + 1023) / 1024;
+  if (g > 256) g = 256;
+  hipLaunchKernelGGL(k_history, dim3((unsigned)g, 4), dim3(1024), 0, s, zre, total, counts, dz);
   return hipGetLastError();
 }
 

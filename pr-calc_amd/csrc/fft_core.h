@@ -224,6 +224,23 @@ constexpr int plan_p() {  // product of the first S radices
   return p;
 }
 
+// Per-stage twiddle tables: stage s with p > 1 owns p*R entries
+// w[k*R + r] = exp(+2 pi i r k / (p R)), k < p, r < R, stored back to back in
+// stage order, so one butterfly reads its R twiddles from consecutive addresses.
+template <class P, int S>
+constexpr int plan_tw_offset() {
+  int off = 0, p = 1;
+  for (int s = 0; s < S; ++s) {
+    if (p > 1) off += p * P::radices[s];
+    p *= P::radices[s];
+  }
+  return off;
+}
+template <class P>
+constexpr int plan_tw_size() {
+  return plan_tw_offset<P, P::S>();
+}
+
 // LDS view of one pencil: element a at base[a * sa]
 struct LdsPencil {
   float2* base;
@@ -231,7 +248,8 @@ struct LdsPencil {
   __device__ __forceinline__ float2& operator[](int a) const { return base[a * sa]; }
 };
 
-// One Stockham stage on registers v (thread index i within the pencil).
+// One Stockham stage on registers v (thread index i within the pencil); tw is
+// the plan's per-stage twiddle table.
 // If LAST, outputs stay in registers (natural order, v[e] = y[i + e*T]);
 // otherwise they are written to LDS at their Stockham positions.
 template <class P, int STAGE, int DIR, bool LAST>
@@ -240,9 +258,10 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const f
   constexpr int L = P::L, E = P::E, T = P::T;
   constexpr int R = P::radices[STAGE];
   constexpr int p = plan_p<P, STAGE>();
-  constexpr int NB = E / R;         // butterflies per thread
-  constexpr int TWS = L / (p * R);  // twiddle table stride
+  constexpr int NB = E / R;  // butterflies per thread
+  constexpr int TWO = plan_tw_offset<P, STAGE>();
   static_assert(E % R == 0, "radix must divide E");
+  (void)L;
 #pragma unroll
   for (int m = 0; m < NB; ++m) {
     float2 u[R];
@@ -251,9 +270,10 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const f
     const int b = i + m * T;
     const int k = (p == 1) ? 0 : (b % p);
     if constexpr (p > 1) {
+      const float2* wk = tw + TWO + k * R;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * k * TWS];
+        float2 w = wk[r];
         if (DIR < 0) w.y = -w.y;
         u[r] = cmul(u[r], w);
       }

@@ -126,14 +126,16 @@ __global__ void __launch_bounds__(P::T* C)
   const int c = tid % C, i = tid / C;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   const long long o = tile % nOuter, cb = tile / nOuter;
-  const TI* ib = in + cb * sCB + o * in_sO + c;
-  TO* ob = out + cb * sCB + o * out_sO + c;
+  // running pointers: element i + e*T sits e*T*sA after element i
+  const TI* ip = in + cb * sCB + o * in_sO + c + (long long)i * in_sA;
+  TO* op = out + cb * sCB + o * out_sO + c + (long long)i * out_sA;
+  const long long istep = (long long)T * in_sA, ostep = (long long)T * out_sA;
   float2 v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = ld(ib, (long long)(i + e * T) * in_sA);
+  for (int e = 0; e < E; ++e, ip += istep) v[e] = ld(ip, 0);
   pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
 #pragma unroll
-  for (int e = 0; e < E; ++e) st(ob, (long long)(i + e * T) * out_sA, cscale(v[e], oscale));
+  for (int e = 0; e < E; ++e, op += ostep) st(op, 0, cscale(v[e], oscale));
 }
 
 // ---------------------------------------------------- inverse z pass (c2r)
@@ -392,6 +394,73 @@ __global__ void __launch_bounds__(256)
 
 // ============================================================ launchers
 #define PRC_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(192) X(256) X(512) X(1024) X(2048)
+
+// per-stage twiddle table of plan P from FP64 sincospi, rounded to float
+template <class P, int STAGE>
+__device__ __forceinline__ bool twst_entry(int q, float2* out) {
+  if constexpr (STAGE >= P::S) {
+    return false;
+  } else {
+    constexpr int p = plan_p<P, STAGE>();
+    constexpr int R = P::radices[STAGE];
+    constexpr int off = plan_tw_offset<P, STAGE>();
+    if (p > 1 && q >= off && q < off + p * R) {
+      const int k = (q - off) / R, r = (q - off) % R;
+      double sn, cs;
+      sincospi(2.0 * (double)(r * k) / (double)(p * R), &sn, &cs);
+      out[q] = make_float2((float)cs, (float)sn);
+      return true;
+    }
+    return twst_entry<P, STAGE + 1>(q, out);
+  }
+}
+
+template <class P>
+__global__ void k_twst(float2* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < plan_tw_size<P>()) twst_entry<P, 0>(q, out);
+}
+
+template <int N>
+static void twst_sizes(int* full, int* half) {
+  *full = plan_tw_size<Plan<N>>();
+  *half = plan_tw_size<Plan<N / 2>>();
+}
+
+template <int N>
+static hipError_t twst_impl(float2* full, float2* half, hipStream_t s) {
+  constexpr int a = plan_tw_size<Plan<N>>(), b = plan_tw_size<Plan<N / 2>>();
+  if (a > 0) hipLaunchKernelGGL((k_twst<Plan<N>>), dim3((a + 255) / 256), dim3(256), 0, s, full);
+  if (b > 0) hipLaunchKernelGGL((k_twst<Plan<N / 2>>), dim3((b + 255) / 256), dim3(256), 0, s, half);
+  return hipGetLastError();
+}
+
+int stage_twiddle_size(int N, bool half) {
+  int a = 0, b = 0;
+  switch (N) {
+#define X(NN)              \
+  case NN:                 \
+    twst_sizes<NN>(&a, &b); \
+    break;
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return -1;
+  }
+  return half ? b : a;
+}
+
+hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return twst_impl<NN>(full, half, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
 
 template <int N>
 static hipError_t fwd_z_impl(const float* in, float2* out, float dinit, const float2* twh, const float2* twf,

@@ -92,8 +92,9 @@ struct prc_d2z_ctx {
   size_t tbytes = 0;  // bytes of the intermediate buffer region
   float* d_zre = nullptr;
   float* d_rb = nullptr;
-  float2* d_tw_full = nullptr;
-  float2* d_tw_half = nullptr;
+  float2* d_tw_full = nullptr;  // exp(+2 pi i k/N), k < N: real-transform glue
+  float2* d_st_full = nullptr;  // per-stage twiddles of the length-N plan
+  float2* d_st_half = nullptr;  // per-stage twiddles of the length-N/2 plan
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
   double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
   int nblk_fz = 0;
@@ -165,12 +166,12 @@ int forward(prc_d2z_ctx* c, float dinit) {
   float2* T0 = c->d_T;
   const TileMap ym = tile_map(N, Axis::kYInPlace), xm = tile_map(N, Axis::kXTtoS);
   PRC_CHECK(c, timed(c, "fwd_z_r2c", [&] {
-              return launch_fwd_z(N, c->d_delta, T0, dinit, c->d_tw_half, c->d_tw_full, s);
+              return launch_fwd_z(N, c->d_delta, T0, dinit, c->d_st_half, c->d_tw_full, s);
             }),
             "forward z pass");
-  PRC_CHECK(c, timed(c, "fwd_y", [&] { return launch_tile(N, -1, T0, T0, ym, c->d_tw_full, s); }),
+  PRC_CHECK(c, timed(c, "fwd_y", [&] { return launch_tile(N, -1, T0, T0, ym, c->d_st_full, s); }),
             "forward y pass");
-  PRC_CHECK(c, timed(c, "fwd_x", [&] { return launch_tile(N, -1, T0, c->d_S, xm, c->d_tw_full, s); }),
+  PRC_CHECK(c, timed(c, "fwd_x", [&] { return launch_tile(N, -1, T0, c->d_S, xm, c->d_st_full, s); }),
             "forward x pass");
   PRC_CHECK(c, timed(c, "unpack", [&] { return launch_unpack(N, c->d_S, c->d_D0, c->d_DN, s); }), "unpack");
   c->have_spectrum = true;
@@ -198,13 +199,13 @@ int plain_radius(prc_d2z_ctx* c, int r, bool barrier) {
               return launch_filter(N, c->d_S, c->d_D0, c->d_DN, T1, dkR, invN3, s);
             }),
             "filter");
-  PRC_CHECK(c, timed(c, "inv_x", [&] { return launch_tile(N, +1, T1, T2, xm, c->d_tw_full, s); }),
+  PRC_CHECK(c, timed(c, "inv_x", [&] { return launch_tile(N, +1, T1, T2, xm, c->d_st_full, s); }),
             "inverse x pass");
-  PRC_CHECK(c, timed(c, "inv_y", [&] { return launch_tile(N, +1, T2, T2, ym, c->d_tw_full, s); }),
+  PRC_CHECK(c, timed(c, "inv_y", [&] { return launch_tile(N, +1, T2, T2, ym, c->d_st_full, s); }),
             "inverse y pass");
   float* dR = reinterpret_cast<float*>(T1);
   PRC_CHECK(c, timed(c, "inv_z_c2r", [&] {
-              return launch_inv_z(N, T2, dR, c->d_tw_half, c->d_tw_full, s);
+              return launch_inv_z(N, T2, dR, c->d_st_half, c->d_tw_full, s);
             }),
             "inverse z pass");
   if (!barrier) return PRC_D2Z_OK;
@@ -243,17 +244,17 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
-                               c->d_tw_full, s);
+                               c->d_st_full, s);
             }),
             "fused x pass");
   PRC_CHECK(c, timed(c, "inv_y", [&] {
-              return half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_tw_full, s)
-                          : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, s);
+              return half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, s)
+                          : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, s);
             }),
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
               return launch_fz(N, c->d_T, half, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->d_rc, r0, nb,
-                               c->d_part, c->nblk_fz, ms.inv, c->d_tw_half, c->d_tw_full, s);
+                               c->d_part, c->nblk_fz, ms.inv, c->d_st_half, c->d_tw_full, s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
@@ -444,7 +445,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_zre, sizeof(float) * c->ncell);
   if (p->RecordBubbleSizes) A_(c->d_rb, sizeof(float) * c->ncell);
   A_(c->d_tw_full, sizeof(float2) * p->N);
-  A_(c->d_tw_half, sizeof(float2) * c->Nh);
+  A_(c->d_st_full, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, false)));
+  A_(c->d_st_half, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, true)));
   A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
   A_(c->d_part, sizeof(double) * (size_t)NS * std::max(c->nblk_fz, 1));
   A_(c->d_sigma_grid, sizeof(float) * NS);
@@ -463,7 +465,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   SetupParams sp{p->Omegam, p->Omegab, p->Omegal, p->h,    p->ns,   p->sigma8, p->w, p->BoxSize,
                  p->zInit,  p->Rmin,   p->Rmax,   p->zeta, p->M_min, p->N,      p->Nscales};
   if ((e = launch_twiddles(c->d_tw_full, p->N, c->stream)) != hipSuccess ||
-      (e = launch_twiddles(c->d_tw_half, c->Nh, c->stream)) != hipSuccess ||
+      (e = launch_stage_twiddles(p->N, c->d_st_full, c->d_st_half, c->stream)) != hipSuccess ||
       (e = launch_setup(sp, c->d_integ, c->d_R, c->d_Rint, c->d_sc, c->d_rc, c->d_sigma, c->stream)) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, std::string("set-up kernels: ") + hipGetErrorString(e));
   c->rc.resize(NS);
@@ -535,9 +537,9 @@ int prc_d2z_generate_delta(prc_d2z_ctx* c, uint64_t seed, int white_noise_only) 
     PRC_CHECK(c, launch_grf_norm(sp, c->d_grfA, s), "field normalisation");
     PRC_CHECK(c, launch_colour(N, c->d_S, c->d_D0, c->d_DN, sp, c->d_grfA, s), "colouring");
     float2* T0 = c->d_T;
-    PRC_CHECK(c, launch_tile(N, +1, c->d_S, T0, tile_map(N, Axis::kXStoT), c->d_tw_full, s), "field x pass");
-    PRC_CHECK(c, launch_tile(N, +1, T0, T0, tile_map(N, Axis::kYInPlace), c->d_tw_full, s), "field y pass");
-    PRC_CHECK(c, launch_inv_z(N, T0, c->d_delta, c->d_tw_half, c->d_tw_full, s), "field z pass");
+    PRC_CHECK(c, launch_tile(N, +1, c->d_S, T0, tile_map(N, Axis::kXStoT), c->d_st_full, s), "field x pass");
+    PRC_CHECK(c, launch_tile(N, +1, T0, T0, tile_map(N, Axis::kYInPlace), c->d_st_full, s), "field y pass");
+    PRC_CHECK(c, launch_inv_z(N, T0, c->d_delta, c->d_st_half, c->d_tw_full, s), "field z pass");
   }
   PRC_CHECK(c, hipStreamSynchronize(s), "field generation");
   c->timing = was_timing;
@@ -651,13 +653,13 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
       const TileMap ym = tile_map(N, Axis::kYInPlace);
       hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
-                               c->d_tw_full, c->stream);
+                               c->d_st_full, c->stream);
       if (e == hipSuccess)
-        e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_tw_full, c->stream)
-                 : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_tw_full, c->stream);
+        e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, c->stream)
+                 : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, c->stream);
       if (e == hipSuccess)
-        e = half ? launch_inv_z_h(N, c->d_T, real, ms.inv, c->d_tw_half, c->d_tw_full, c->stream)
-                 : launch_inv_z(N, c->d_T, real, c->d_tw_half, c->d_tw_full, c->stream);
+        e = half ? launch_inv_z_h(N, c->d_T, real, ms.inv, c->d_st_half, c->d_tw_full, c->stream)
+                 : launch_inv_z(N, c->d_T, real, c->d_st_half, c->d_tw_full, c->stream);
       if (e != hipSuccess) rc = c->hip_fail(e, "smoothed field");
     }
   }
