@@ -150,6 +150,10 @@ int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */)
  * through the context's transform path. */
 int prc_d2z_smoothed_field(prc_d2z_ctx* ctx, int r, float* out);
 
+/* Replay prc_d2z_run as one captured hipGraph (on by default; the graph is
+ * captured on the first run of the context).  Timing mode runs eagerly. */
+int prc_d2z_set_graph(prc_d2z_ctx* ctx, int enable);
+
 /* Per-kernel HIP-event timing of prc_d2z_run (off by default). */
 int prc_d2z_set_timing(prc_d2z_ctx* ctx, int enable);
 int prc_d2z_get_timing(prc_d2z_ctx* ctx, prc_d2z_timing* out);

@@ -33,6 +33,19 @@ __device__ __forceinline__ float2 ld(const __half2* p, long long i) { return __h
 __device__ __forceinline__ void st(float2* p, long long i, float2 v) { p[i] = v; }
 __device__ __forceinline__ void st(__half2* p, long long i, float2 v) { p[i] = __float22half2_rn(v); }
 
+// Offset of element k = i + e*T of z row r in the T layout ([k/C][rows][k%C]).
+// When C divides T the offset advances by a constant per e.
+template <class P, int C>
+__device__ __forceinline__ long long zrow_offset(int i, int e, long long r, long long nrows) {
+  constexpr int T = P::T;
+  if constexpr (T % C == 0) {
+    return ((long long)(i / C) * nrows + r) * C + (i % C) + (long long)e * (T / C) * nrows * C;
+  } else {
+    const int k = i + e * T;
+    return ((long long)(k / C) * nrows + r) * C + (k % C);
+  }
+}
+
 // ---------------------------------------------------- real-transform glue
 // Forward untangling of Z = FFT_L(x[2n] + i x[2n+1]) into the packed half
 // spectrum: X[k] = (Z[k] + conj Z[L-k])/2 - i exp(-2 pi i k/N) (Z[k] - conj Z[L-k])/2,
@@ -104,10 +117,7 @@ __global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, flo
   r2c_post<P>(v, m, i, tw_full);
   if (valid) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int k = i + e * T;
-      out[((long long)(k / C) * nrows + row) * C + (k % C)] = v[e];
-    }
+    for (int e = 0; e < E; ++e) out[zrow_offset<P, C>(i, e, row, nrows)] = v[e];
   }
 }
 
@@ -154,10 +164,7 @@ __global__ void __launch_bounds__(256) k_inv_z(const TS* __restrict__ in, float*
   const long long r = valid ? row : 0;
   float2 v[E], m[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int k = i + e * T;
-    v[e] = ld(in, ((long long)(k / C) * nrows + r) * C + (k % C));
-  }
+  for (int e = 0; e < E; ++e) v[e] = ld(in, zrow_offset<P, C>(i, e, r, nrows));
   mirror_lanes<P>(v, m, i);
   c2r_pre<P>(v, m, i, tw_full);
   pencil_fft<P, +1>(v, i, tw_half, LdsPencil{smem + rr * L, 1});
@@ -337,10 +344,7 @@ __global__ void __launch_bounds__(256)
       const TS* tp = Tbuf + (long long)q * plane;
       float2 v[E], m[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int k = i + e * T;
-        v[e] = ld(tp, ((long long)(k / C) * nrows + r) * C + (k % C));
-      }
+      for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
       mirror_lanes<P>(v, m, i);
       c2r_pre<P>(v, m, i, tw_full);
       pencil_fft<P, +1>(v, i, tw_half, lds);

@@ -115,6 +115,8 @@ struct prc_d2z_ctx {
   size_t bytes = 0;
   bool timing = false;
   KernelTimer timer;
+  bool use_graph = true;
+  hipGraphExec_t graph_exec = nullptr;
   std::vector<void*> allocs;
 
   int fail(int code, const std::string& msg) {
@@ -260,17 +262,9 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   return PRC_D2Z_OK;
 }
 
-int run_all(prc_d2z_ctx* c) {
-  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+// All launches of one run (forward transform through the last radius).
+int enqueue_run(prc_d2z_ctx* c) {
   hipStream_t s = c->stream;
-  if (c->timing) {
-    c->timer.reset();
-    if (!c->timer.t0) {
-      (void)hipEventCreate(&c->timer.t0);
-      (void)hipEventCreate(&c->timer.t1);
-    }
-    (void)hipEventRecord(c->timer.t0, s);
-  }
   int rc = forward(c, c->sc.DInit);
   if (rc) return rc;
   PRC_CHECK(c, launch_fill(c->d_zre, c->ncell, 0.0f, s), "zreion init");
@@ -288,7 +282,50 @@ int run_all(prc_d2z_ctx* c) {
     }
     PRC_CHECK(c, launch_sigma_finish(c->d_part, c->nblk_fz, NS, invN3, c->d_sigma_grid, s), "sigma_grid");
   }
-  if (c->timing) (void)hipEventRecord(c->timer.t1, s);
+  return PRC_D2Z_OK;
+}
+
+// One run.  With timing on, launches are enqueued directly with events around
+// each; otherwise the whole sequence is captured once into a hipGraph (all
+// buffers and launch arguments are fixed for the context's lifetime) and
+// replayed.
+int run_all(prc_d2z_ctx* c) {
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  hipStream_t s = c->stream;
+  int rc = PRC_D2Z_OK;
+  if (c->timing) {
+    c->timer.reset();
+    if (!c->timer.t0) {
+      (void)hipEventCreate(&c->timer.t0);
+      (void)hipEventCreate(&c->timer.t1);
+    }
+    (void)hipEventRecord(c->timer.t0, s);
+    rc = enqueue_run(c);
+    (void)hipEventRecord(c->timer.t1, s);
+  } else if (c->use_graph) {
+    if (!c->graph_exec) {
+      PRC_CHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture");
+      rc = enqueue_run(c);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return c->hip_fail(e, "graph capture");
+      e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        c->graph_exec = nullptr;
+        return c->hip_fail(e, "graph instantiate");
+      }
+    }
+    PRC_CHECK(c, hipGraphLaunch(c->graph_exec, s), "graph launch");
+  } else {
+    rc = enqueue_run(c);
+  }
+  if (rc) return rc;
+  c->have_spectrum = true;
   c->have_result = true;
   return PRC_D2Z_OK;
 }
@@ -354,6 +391,7 @@ void prc_d2z_destroy(prc_d2z_ctx* c) {
   (void)hipSetDevice(c->p.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->timer.destroy();
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (void* q : c->allocs) (void)hipFree(q);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipSetDevice(dev);
@@ -672,6 +710,12 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
 int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {
   if (!c) return PRC_D2Z_ERR_BAD_ARG;
   c->timing = enable != 0;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_set_graph(prc_d2z_ctx* c, int enable) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  c->use_graph = enable != 0;
   return PRC_D2Z_OK;
 }
 

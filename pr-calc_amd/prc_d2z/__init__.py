@@ -66,7 +66,7 @@ EXPORTS = [
     "prc_d2z_run_async", "prc_d2z_sync", "prc_d2z_get_zreion", "prc_d2z_get_rbubble",
     "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
     "prc_d2z_smoothed_field", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
-    "prc_d2z_generate_delta", "prc_d2z_get_delta",
+    "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
 ]
 
 _lib = None
@@ -107,6 +107,7 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_device_bytes": (C.c_size_t, [vp]),
         "prc_d2z_generate_delta": (C.c_int, [vp, C.c_uint64, C.c_int]),
         "prc_d2z_get_delta": (C.c_int, [vp, fp, C.c_int]),
+        "prc_d2z_set_graph": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -234,6 +235,9 @@ class D2Z:
         out = np.empty((N, N, N), dtype=np.float32)
         self._check(self.lib.prc_d2z_smoothed_field(self._ctx, r, out))
         return out
+
+    def set_graph(self, on: bool = True):
+        self._check(self.lib.prc_d2z_set_graph(self._ctx, int(on)))
 
     def set_timing(self, on: bool = True):
         self._check(self.lib.prc_d2z_set_timing(self._ctx, int(on)))

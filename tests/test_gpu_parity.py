@@ -261,3 +261,31 @@ def test_c2_size_512(mode):
         assert err <= 5e-5
         # sigma_grid of the run equals the rms of the smoothed field
         assert tab[r, 2] == pytest.approx(np.sqrt(np.mean(ref ** 2)), rel=1e-5)
+
+
+@pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, 2])
+def test_graph_replay_matches_eager(mode):
+    """The captured run is replayed several times, with eager library work and
+    synchronisations in between; every replay equals the eager result bit for bit."""
+    N = 64
+    lp = lib_params(N, nscales=9, mode=mode, rpl=4, rbub=1)
+    announce(f"graph replay N={N} mode={mode}")
+    with D2Z(lp) as d:
+        d.set_delta(field(N))
+        d.set_graph(False)
+        d.run()
+        z0, rb0, tab0 = d.zreion(), d.rbubble(), d.radius_table()
+        d.set_graph(True)
+        for k in range(3):
+            d.run()
+            assert np.array_equal(d.zreion(), z0), k
+            assert np.array_equal(d.rbubble(), rb0), k
+            assert np.array_equal(d.radius_table(), tab0), k
+            d.smoothed_field(k)        # eager work between replays
+            d.history()
+        d.generate_delta(5)            # new input in the same buffer: replay must see it
+        d.run()
+        z5 = d.zreion()
+        d.set_graph(False)
+        d.run()
+        assert np.array_equal(d.zreion(), z5)
