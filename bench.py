@@ -151,6 +151,15 @@ def main():
             "path_achieved": value * B_ALG_PATH / 1e9,
             "path_frac": value * B_ALG_PATH / 1e9 / HBM_PEAK_GBS}
 
+    # the same steps replayed as the captured hipGraph (no per-launch events)
+    d.run()
+    d.sync()
+    g0 = time.perf_counter()
+    for _ in range(args.steps):
+        d.run_async()
+    d.sync()
+    graph_ms = (time.perf_counter() - g0) / args.steps * 1e3
+
     if args.history:
         h = d.history()
         print(f"history: {int(h.sum())} cells counted", file=sys.stderr)
@@ -173,6 +182,7 @@ def main():
         "config": {"workload": f"{N}^3 box {box:g} Mpc/h, {NS} radii 1 cell..{rmax:g} Mpc/h, "
                                f"zeta 40, Mmin 1e8 Msun/h, GRF seed 1", "mode": args.mode,
                    "radii_per_launch": nb},
+        "graph_ms_per_step": graph_ms,
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
         "cpu_baseline": cpu,
