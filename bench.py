@@ -37,7 +37,11 @@ WORKLOADS = {
     "c1_128": (128, 200.0, 32, 50.0),
     "c2_512": (512, 1000.0, 64, 80.0),
     "c3_1024": (1024, 2000.0, 96, 120.0),
+    "c5_2048": (2048, 4000.0, 128, 150.0),
 }
+# 48 uneven boxes: 16 x 128^3 (200 Mpc/h), 16 x 192^3 (300), 16 x 256^3 (400),
+# distinct seeds, 48 radii each from one cell to 50 Mpc/h
+MULTIBOX = [(128, 200.0), (192, 300.0), (256, 400.0)]
 MODES = {"plain": 0, "fused": 1, "mixed": 2}
 
 
@@ -98,7 +102,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="fused", choices=list(MODES))
-    ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS))
+    ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS) + ["c4_multibox"])
     ap.add_argument("--radii-per-launch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -114,6 +118,9 @@ def main():
 
     from prc_d2z import D2Z, D2ZParams
 
+    if args.workload == "c4_multibox":
+        multibox(args)
+        return
     N, box, NS, rmax = WORKLOADS[args.workload]
     lp = D2ZParams(N=N, BoxSize=box, Nscales=NS, Rmax=rmax, zeta=40.0, M_min=1.4706e8,
                    mode=MODES[args.mode], radii_per_launch=args.radii_per_launch)
@@ -186,6 +193,60 @@ def main():
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def multibox(args):
+    """48 boxes of three sizes submitted together (each box's captured run on
+    its own stream) against the same boxes run one after another on the plain
+    path.  value = aggregate cell-radius updates per second of the batch."""
+    from prc_d2z import D2Z, D2ZParams, run_many
+
+    mode = MODES[args.mode]
+    boxes, seed, cells = [], 1, 0
+    for N, box in MULTIBOX:
+        for _ in range(16):
+            d = D2Z(D2ZParams(N=N, BoxSize=box, Nscales=48, Rmax=50.0, zeta=40.0, M_min=1.4706e8, mode=mode))
+            d.generate_delta(seed)
+            seed += 1
+            boxes.append(d)
+            cells += N ** 3
+    for _ in range(max(1, args.warmup)):
+        run_many(boxes)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_many(boxes)
+    batched = (time.perf_counter() - t0) / args.steps
+    # reference point: the same 48 boxes as separate plain launches, one after another
+    plain = []
+    for b in boxes:
+        p = b.params.copy(mode=MODES["plain"])
+        plain.append(p)
+    seq_t = 0.0
+    seed = 1
+    for p in plain:
+        d = D2Z(p)
+        d.generate_delta(seed)
+        seed += 1
+        d.set_graph(False)
+        d.run()
+        t1 = time.perf_counter()
+        d.run()
+        seq_t += time.perf_counter() - t1
+        d.close()
+    for b in boxes:
+        b.close()
+    value = cells * 48 / batched
+    out = {
+        "metric": "aggregate cell*radius updates/s, 48 uneven boxes (delta2zreion)",
+        "value": value, "unit": "cell*radius/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": batched * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.mode != "mixed" else "f32/f16", "data": "synthetic",
+        "config": {"workload": "16x128^3 (200 Mpc/h) + 16x192^3 (300) + 16x256^3 (400), 48 radii each "
+                               "1 cell..50 Mpc/h, distinct GRF seeds", "mode": args.mode},
+        "separate_plain_ms": seq_t * 1e3,
+        "speedup_vs_separate_plain": seq_t / batched,
     }
     print(json.dumps(out))
 

@@ -150,6 +150,26 @@ int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */)
  * through the context's transform path. */
 int prc_d2z_smoothed_field(prc_d2z_ctx* ctx, int r, float* out);
 
+/* Launch shape of the fused modes: radii per fused batch (1..max_radii, bounded
+ * by the intermediate buffers allocated at create) and the grid of the fused
+ * z pass (1..4096 blocks). */
+int prc_d2z_set_launch_config(prc_d2z_ctx* ctx, int radii_per_launch, int z_blocks);
+int prc_d2z_get_launch_config(const prc_d2z_ctx* ctx, int* radii_per_launch, int* z_blocks, int* max_radii);
+
+typedef struct {
+  int radii_per_launch;
+  int z_blocks;
+  double ms_per_radius;
+} prc_d2z_tuning;
+/* Autotuner: times the fused batches of the first radii for each candidate
+ * launch shape on the current field, keeps and returns the fastest.  The host
+ * persists the choice (see prc_d2z.tuning in the Python binding). */
+int prc_d2z_autotune(prc_d2z_ctx* ctx, prc_d2z_tuning* out);
+
+/* Runs several contexts (boxes of any size on the same device) at once: every
+ * context's captured run is submitted on its own stream, then all are awaited. */
+int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n);
+
 /* Replay prc_d2z_run as one captured hipGraph (on by default; the graph is
  * captured on the first run of the context).  Timing mode runs eagerly. */
 int prc_d2z_set_graph(prc_d2z_ctx* ctx, int enable);

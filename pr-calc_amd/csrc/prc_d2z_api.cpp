@@ -65,6 +65,7 @@ struct KernelTimer {
 };
 
 constexpr int kDefaultRadiiPerLaunch = 8;
+constexpr int kMaxZBlocks = 4096;  // largest grid of the fused z pass
 constexpr int kMomentBlocks = 1024;
 
 bool supported_n(int N) {
@@ -88,7 +89,8 @@ struct prc_d2z_ctx {
   float2* d_D0 = nullptr;
   float2* d_DN = nullptr;
   float2* d_T = nullptr;
-  int nbatch = 1;     // radii per fused batch
+  int nbatch = 1;      // radii per fused batch
+  int nbatch_cap = 1;  // most radii per batch the intermediate buffers hold
   size_t tbytes = 0;  // bytes of the intermediate buffer region
   float* d_zre = nullptr;
   float* d_rb = nullptr;
@@ -463,13 +465,20 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
   const size_t tstore = (p->mode == PRC_D2Z_FUSED_MIXED) ? sizeof(__half2) : sizeof(float2);
-  c->tbytes = std::max((size_t)2 * sizeof(float2) * c->nspec, tstore * c->nspec * c->nbatch);
-  size_t need = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) + sizeof(float2) * c->nspec +
-                c->tbytes + sizeof(float2) * 2 * (size_t)p->N * p->N;
+  const size_t fixed = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) + sizeof(float2) * c->nspec +
+                       sizeof(float2) * 2 * (size_t)p->N * p->N + (64u << 20);
   size_t freeb = 0, totalb = 0;
-  if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && need + (64u << 20) > freeb)
+  const bool have_info = hipMemGetInfo(&freeb, &totalb) == hipSuccess;
+  auto tbytes_for = [&](int nb) { return std::max((size_t)2 * sizeof(float2) * c->nspec, tstore * c->nspec * nb); };
+  // default batch: the largest radii-per-launch (<= 8) whose intermediates fit
+  if (p->mode != PRC_D2Z_PLAIN_FP32 && !p->radii_per_launch && have_info)
+    while (c->nbatch > 1 && fixed + tbytes_for(c->nbatch) > freeb) --c->nbatch;
+  c->tbytes = tbytes_for(c->nbatch);
+  const size_t need = fixed + c->tbytes;
+  if (have_info && need > freeb)
     return bail(PRC_D2Z_ERR_OOM, "device memory: need " + std::to_string(need >> 20) + " MiB, free " +
                                      std::to_string(freeb >> 20) + " MiB");
+  c->nbatch_cap = (p->mode == PRC_D2Z_PLAIN_FP32) ? 1 : (int)std::min<size_t>(8, c->tbytes / (tstore * c->nspec));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
   hipError_t e = hipSuccess;
@@ -486,7 +495,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_st_full, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, false)));
   A_(c->d_st_half, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, true)));
   A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
-  A_(c->d_part, sizeof(double) * (size_t)NS * std::max(c->nblk_fz, 1));
+  A_(c->d_part, sizeof(double) * (size_t)NS * kMaxZBlocks);
   A_(c->d_sigma_grid, sizeof(float) * NS);
   A_(c->d_sigma, sizeof(float) * NS);
   A_(c->d_rc, sizeof(RadiusConst) * NS);
@@ -710,6 +719,104 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
 int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {
   if (!c) return PRC_D2Z_ERR_BAD_ARG;
   c->timing = enable != 0;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_set_launch_config(prc_d2z_ctx* c, int radii_per_launch, int z_blocks) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  if (c->p.mode == PRC_D2Z_PLAIN_FP32) return c->fail(PRC_D2Z_ERR_STATE, "launch configuration applies to fused modes");
+  if (radii_per_launch < 1 || radii_per_launch > c->nbatch_cap)
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "radii_per_launch must be in 1.." + std::to_string(c->nbatch_cap));
+  if (z_blocks < 1 || z_blocks > kMaxZBlocks)
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "z_blocks must be in 1.." + std::to_string(kMaxZBlocks));
+  c->nbatch = std::min(radii_per_launch, c->p.Nscales);
+  c->nblk_fz = z_blocks;
+  if (c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_launch_config(const prc_d2z_ctx* c, int* radii_per_launch, int* z_blocks, int* max_radii) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  if (radii_per_launch) *radii_per_launch = c->nbatch;
+  if (z_blocks) *z_blocks = c->nblk_fz;
+  if (max_radii) *max_radii = c->nbatch_cap;
+  return PRC_D2Z_OK;
+}
+
+// Times the fused batches covering the first min(Nscales, 8) radii for each
+// candidate (radii per launch in {cap, cap/2, .., 1} x z grid in {512 .. 4096})
+// with HIP events and keeps the fastest per radius.  Needs a density field;
+// leaves zreion undefined until the next run.
+int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
+  if (!c || !out) return PRC_D2Z_ERR_BAD_ARG;
+  if (c->p.mode == PRC_D2Z_PLAIN_FP32) return c->fail(PRC_D2Z_ERR_STATE, "autotune applies to fused modes");
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  (void)hipSetDevice(c->p.device);
+  const bool was_timing = c->timing;
+  c->timing = false;
+  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c, c->sc.DInit);
+  if (rc) {
+    c->timing = was_timing;
+    return rc;
+  }
+  const int nr = std::min(c->p.Nscales, 8);
+  std::vector<int> nbs;
+  for (int nb = c->nbatch_cap; nb >= 1; nb = (nb > 1) ? nb / 2 : 0) nbs.push_back(nb);
+  const int grids[] = {512, 1024, 2048, 4096};
+  hipEvent_t a = nullptr, b = nullptr;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  double best = 1e30;
+  int best_nb = c->nbatch, best_grid = c->nblk_fz;
+  const int keep_nb = c->nbatch, keep_grid = c->nblk_fz;
+  for (int nb : nbs) {
+    for (int g : grids) {
+      c->nbatch = nb;
+      c->nblk_fz = g;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up
+        (void)hipEventRecord(a, c->stream);
+        for (int r0 = 0; r0 < nr && !rc; r0 += nb) rc = fused_batch(c, r0, std::min(nb, nr - r0));
+        (void)hipEventRecord(b, c->stream);
+      }
+      if (rc) break;
+      (void)hipEventSynchronize(b);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      const double per = ms / nr;
+      if (per < best) {
+        best = per;
+        best_nb = nb;
+        best_grid = g;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->timing = was_timing;
+  c->have_result = false;
+  c->nbatch = keep_nb;
+  c->nblk_fz = keep_grid;
+  if (rc) return rc;
+  out->radii_per_launch = best_nb;
+  out->z_blocks = best_grid;
+  out->ms_per_radius = best;
+  return prc_d2z_set_launch_config(c, best_nb, best_grid);
+}
+
+int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n) {
+  if (!ctxs || n < 0) return PRC_D2Z_ERR_BAD_ARG;
+  for (int k = 0; k < n; ++k) {
+    int rc = prc_d2z_run_async(ctxs[k]);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < n; ++k) {
+    int rc = prc_d2z_sync(ctxs[k]);
+    if (rc) return rc;
+  }
   return PRC_D2Z_OK;
 }
 

@@ -54,6 +54,10 @@ class CRadiusRow(C.Structure):
     _fields_ = [("R_mpch", C.c_float), ("sigma", C.c_float), ("sigma_grid", C.c_float)]
 
 
+class CTuning(C.Structure):
+    _fields_ = [("radii_per_launch", C.c_int), ("z_blocks", C.c_int), ("ms_per_radius", C.c_double)]
+
+
 class CTiming(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("nkernels", C.c_int),
                 ("name", (C.c_char * 32) * MAX_KERNELS),
@@ -67,6 +71,7 @@ EXPORTS = [
     "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
     "prc_d2z_smoothed_field", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
+    "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
 ]
 
 _lib = None
@@ -108,6 +113,11 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_generate_delta": (C.c_int, [vp, C.c_uint64, C.c_int]),
         "prc_d2z_get_delta": (C.c_int, [vp, fp, C.c_int]),
         "prc_d2z_set_graph": (C.c_int, [vp, C.c_int]),
+        "prc_d2z_set_launch_config": (C.c_int, [vp, C.c_int, C.c_int]),
+        "prc_d2z_get_launch_config": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                C.POINTER(C.c_int)]),
+        "prc_d2z_autotune": (C.c_int, [vp, C.POINTER(CTuning)]),
+        "prc_d2z_run_many": (C.c_int, [C.POINTER(vp), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -236,6 +246,35 @@ class D2Z:
         self._check(self.lib.prc_d2z_smoothed_field(self._ctx, r, out))
         return out
 
+    def set_launch_config(self, radii_per_launch: int, z_blocks: int):
+        self._check(self.lib.prc_d2z_set_launch_config(self._ctx, radii_per_launch, z_blocks))
+
+    def launch_config(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.prc_d2z_get_launch_config(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return {"radii_per_launch": a.value, "z_blocks": b.value, "max_radii": c.value}
+
+    def autotune(self, cache: str | os.PathLike | None = None) -> dict:
+        """Pick the launch shape with the library's autotuner.  With ``cache`` (a
+        JSON file) a stored choice for this (device, N, mode) is reused, and a new
+        one is stored."""
+        import json
+        key = f"{self.params.device}:N{self.N}:mode{self.params.mode}"
+        if cache is not None and Path(cache).exists():
+            table = json.loads(Path(cache).read_text())
+            if key in table:
+                t = table[key]
+                self.set_launch_config(t["radii_per_launch"], t["z_blocks"])
+                return dict(t, cached=True)
+        t = CTuning()
+        self._check(self.lib.prc_d2z_autotune(self._ctx, C.byref(t)))
+        res = {"radii_per_launch": t.radii_per_launch, "z_blocks": t.z_blocks, "ms_per_radius": t.ms_per_radius}
+        if cache is not None:
+            table = json.loads(Path(cache).read_text()) if Path(cache).exists() else {}
+            table[key] = res
+            Path(cache).write_text(json.dumps(table, indent=1, sort_keys=True))
+        return dict(res, cached=False)
+
     def set_graph(self, on: bool = True):
         self._check(self.lib.prc_d2z_set_graph(self._ctx, int(on)))
 
@@ -250,3 +289,15 @@ class D2Z:
 
     def device_bytes(self) -> int:
         return int(self.lib.prc_d2z_device_bytes(self._ctx))
+
+
+def run_many(boxes) -> None:
+    """Run several D2Z contexts at once (each on its own stream)."""
+    boxes = list(boxes)
+    if not boxes:
+        return
+    lib = boxes[0].lib
+    arr = (C.c_void_p * len(boxes))(*[b._ctx.value for b in boxes])
+    rc = lib.prc_d2z_run_many(arr, len(boxes))
+    if rc:
+        raise D2ZError(rc, "run_many: " + " | ".join(lib.prc_d2z_last_error(b._ctx).decode() for b in boxes))

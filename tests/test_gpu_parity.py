@@ -289,3 +289,49 @@ def test_graph_replay_matches_eager(mode):
         d.set_graph(False)
         d.run()
         assert np.array_equal(d.zreion(), z5)
+
+
+def test_autotune_and_launch_config(tmp_path):
+    N = 128
+    lp = lib_params(N, nscales=12)
+    announce("autotune N=128")
+    with D2Z(lp) as d:
+        d.set_delta(field(N))
+        d.run()
+        z_default = d.zreion()
+        cfg0 = d.launch_config()
+        assert cfg0["radii_per_launch"] == 8 and cfg0["max_radii"] == 8
+        cache = tmp_path / "tune.json"
+        t = d.autotune(cache)
+        assert 1 <= t["radii_per_launch"] <= 8 and 512 <= t["z_blocks"] <= 4096 and not t["cached"]
+        d.run()
+        assert np.array_equal(d.zreion(), z_default)   # launch shape never changes results
+        assert d.autotune(cache)["cached"]
+        d.set_launch_config(3, 700)
+        d.run()
+        assert np.array_equal(d.zreion(), z_default)
+        with pytest.raises(D2ZError):
+            d.set_launch_config(9, 1024)
+
+
+def test_run_many_matches_single_runs():
+    from prc_d2z import run_many
+    cases = [(64, 3), (128, 4), (192, 5), (128, 6)]
+    boxes, ref = [], []
+    announce("run_many")
+    for N, seed in cases:
+        lp = lib_params(N, nscales=6)
+        d = D2Z(lp)
+        d.generate_delta(seed)
+        d.run()
+        ref.append(d.zreion())
+        boxes.append(d)
+    for d in boxes:              # clobber results, then recompute all at once
+        d.generate_delta(99)
+        d.run()
+    for d, (N, seed) in zip(boxes, cases):
+        d.generate_delta(seed)
+    run_many(boxes)
+    for d, z in zip(boxes, ref):
+        assert np.array_equal(d.zreion(), z)
+        d.close()
