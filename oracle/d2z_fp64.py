@@ -353,17 +353,25 @@ def kmag_grid(N, box_mpc):
     return np.sqrt(kk[:, None, None] ** 2 + kk[None, :, None] ** 2 + kz[None, None, :] ** 2)
 
 
-def forward(delta, DInit):
-    """delta /= DInit then unnormalised r2c (main_d2z.C:57-59,67)."""
-    return np.fft.rfftn(np.asarray(delta, dtype=np.float64) / f64(DInit))
+def forward(delta, DInit, workers=None):
+    """delta /= DInit then unnormalised r2c (main_d2z.C:57-59,67).  With
+    ``workers`` the transform runs on scipy.fft's threaded FFT (large boxes)."""
+    x = np.asarray(delta, dtype=np.float64) / f64(DInit)
+    if workers:
+        import scipy.fft
+        return scipy.fft.rfftn(x, workers=workers)
+    return np.fft.rfftn(x)
 
 
-def smooth(dhat, kmag, R):
+def smooth(dhat, kmag, R, workers=None):
     """Filtered real field delta_R = c2r(W(kR) dhat)/N^3 with k=0 zeroed
     (delta2zreion_d2z.C:58-93)."""
     N = dhat.shape[0]
     W = tophat_W(kmag * f64(R))
     W.flat[0] = 0.0
+    if workers:
+        import scipy.fft
+        return scipy.fft.irfftn(dhat * W, s=(N, N, N), axes=(0, 1, 2), workers=workers)
     return np.fft.irfftn(dhat * W, s=(N, N, N), axes=(0, 1, 2))
 
 

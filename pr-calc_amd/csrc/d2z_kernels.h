@@ -66,7 +66,10 @@ hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, con
 // fused passes; Tbuf holds float2 (half = false) or __half2 (half = true)
 hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
                      float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
-int fz_blocks(int N);
+// blocks of the fused z pass: the device's resident capacity for that kernel,
+// at most kFzMaxBlocks and at most one per 256-thread row group
+constexpr int kFzMaxBlocks = 4096;
+int fz_blocks(int N, bool half, bool rb);
 hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
                      int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
                      hipStream_t s);

@@ -18,6 +18,9 @@
 
 namespace prc {
 
+// floor(log2(n)) for n >= 1
+constexpr int ilog2(int n) { return n > 1 ? 1 + ilog2(n / 2) : 0; }
+
 // ---------------------------------------------------------------- complex
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -248,13 +251,56 @@ struct LdsPencil {
   __device__ __forceinline__ float2& operator[](int a) const { return base[a * sa]; }
 };
 
+// LDS layout of pencils that each own a contiguous row (the z passes): row rr
+// starts at rr * (L + PAD) and element a sits at a ^ ((a >> SH) & MK), an XOR
+// swizzle inside aligned 32-element blocks.  The early Stockham stages write
+// with a stride of R elements, which without it lands all lanes of a wave on a
+// few banks; SH/MK/PAD are chosen per length so that every write and gather of
+// every stage touches distinct banks for 16 (32-bank) or 32 (64-bank) lanes of
+// float2 per cycle.  L must be a multiple of 32 when MK > 15.
+template <int L>
+struct RowSwizzle { static constexpr int SH = 0, MK = 0, PAD = 0; };
+template <> struct RowSwizzle<32> { static constexpr int SH = 3, MK = 15, PAD = 4; };
+template <> struct RowSwizzle<64> { static constexpr int SH = 3, MK = 15, PAD = 8; };
+template <> struct RowSwizzle<96> { static constexpr int SH = 0, MK = 0, PAD = 4; };
+template <> struct RowSwizzle<128> { static constexpr int SH = 4, MK = 15, PAD = 8; };
+template <> struct RowSwizzle<192> { static constexpr int SH = 0, MK = 0, PAD = 8; };
+template <> struct RowSwizzle<256> { static constexpr int SH = 4, MK = 15, PAD = 16; };
+template <> struct RowSwizzle<512> { static constexpr int SH = 3, MK = 31, PAD = 0; };
+template <> struct RowSwizzle<1024> { static constexpr int SH = 4, MK = 31, PAD = 0; };
+
+// LDS view of one column of a tile whose columns are interleaved (element a
+// of column c at a * CF + c): rows are padded by one row of CF every 2^SH rows,
+// so that two pencils 2^SH rows apart (the stride of a radix-2^SH first stage)
+// sit half a bank cycle apart.  Holds (L + L / 2^SH) * CF elements.
+template <int SH>
+struct LdsColPad {
+  float2* base;  // column start
+  int sa;        // CF
+  __device__ __forceinline__ float2& operator[](int a) const { return base[(a + (a >> SH)) * sa]; }
+};
+
+template <int L>
+struct LdsRow {
+  using Sw = RowSwizzle<L>;
+  static constexpr int STRIDE = L + Sw::PAD;  // float2 per row
+  float2* base;
+  __device__ __forceinline__ float2& operator[](int a) const {
+    if constexpr (Sw::MK == 0) {
+      return base[a];
+    } else {
+      return base[a ^ ((a >> Sw::SH) & Sw::MK)];
+    }
+  }
+};
+
 // One Stockham stage on registers v (thread index i within the pencil); tw is
 // the plan's per-stage twiddle table.
 // If LAST, outputs stay in registers (natural order, v[e] = y[i + e*T]);
 // otherwise they are written to LDS at their Stockham positions.
-template <class P, int STAGE, int DIR, bool LAST>
+template <class P, int STAGE, int DIR, bool LAST, class V>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const float2* __restrict__ tw,
-                                               LdsPencil lds) {
+                                               V lds) {
   constexpr int L = P::L, E = P::E, T = P::T;
   constexpr int R = P::radices[STAGE];
   constexpr int p = plan_p<P, STAGE>();
@@ -290,8 +336,8 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const f
   }
 }
 
-template <class P>
-__device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, LdsPencil lds) {
+template <class P, class V>
+__device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, V lds) {
 #pragma unroll
   for (int e = 0; e < P::E; ++e) v[e] = lds[i + e * P::T];
 }
@@ -299,9 +345,8 @@ __device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, LdsPencil l
 // Full pencil transform: registers in natural layout -> registers in natural
 // layout.  All threads of the block must call it (it contains barriers); the
 // LDS region may be reused right after it returns.
-template <class P, int DIR, int STAGE = 0>
-__device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw,
-                                           LdsPencil lds) {
+template <class P, int DIR, int STAGE = 0, class V>
+__device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw, V lds) {
   if constexpr (STAGE == P::S - 1) {
     stockham_stage<P, STAGE, DIR, true>(v, i, tw, lds);
   } else {

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, flo
                                                const float2* __restrict__ tw_full) {
   constexpr int L = P::L, T = P::T, E = P::E;
   constexpr int ROWS = 256 / T;
-  __shared__ float2 smem[ROWS * L];
+  __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
   const long long row = (long long)blockIdx.x * ROWS + rr;
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, flo
     const float2 a = src[i + e * T];
     v[e] = make_float2(a.x / dinit, a.y / dinit);  // delta /= DInit (main_d2z.C:57-59)
   }
-  pencil_fft<P, -1>(v, i, tw_half, LdsPencil{smem + rr * L, 1});
+  pencil_fft<P, -1>(v, i, tw_half, LdsRow<L>{smem + rr * LdsRow<L>::STRIDE});
   mirror_lanes<P>(v, m, i);
   r2c_post<P>(v, m, i, tw_full);
   if (valid) {
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_inv_z(const TS* __restrict__ in, float*
                                                const float2* __restrict__ tw_full) {
   constexpr int L = P::L, T = P::T, E = P::E;
   constexpr int ROWS = 256 / T;
-  __shared__ float2 smem[ROWS * L];
+  __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
   const long long row = (long long)blockIdx.x * ROWS + rr;
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_inv_z(const TS* __restrict__ in, float*
   for (int e = 0; e < E; ++e) v[e] = ld(in, zrow_offset<P, C>(i, e, r, nrows));
   mirror_lanes<P>(v, m, i);
   c2r_pre<P>(v, m, i, tw_full);
-  pencil_fft<P, +1>(v, i, tw_half, LdsPencil{smem + rr * L, 1});
+  pencil_fft<P, +1>(v, i, tw_half, LdsRow<L>{smem + rr * LdsRow<L>::STRIDE});
   if (valid) {
     float2* dst = reinterpret_cast<float2*>(out + row * (2 * L));
 #pragma unroll
@@ -228,7 +228,9 @@ __global__ void k_filter(const float2* __restrict__ S, const float2* __restrict_
 // W depends on kx only through |fx| (fx = kx or kx - N), so per radius the
 // block first tabulates W for its CF columns and the N/2 + 1 values of |fx|
 // in LDS (half the filter evaluations of a per-element loop), plus the
-// kz = N/2 column for the packed slot.
+// kz = N/2 column for the packed slot.  The table is laid out [|fx|][column]
+// so that the lanes of a wave (consecutive columns, then consecutive kx) read
+// consecutive words.
 template <class P, int C, int CF, class TS>
 __global__ void __launch_bounds__(P::T* CF)
     k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
@@ -240,11 +242,14 @@ __global__ void __launch_bounds__(P::T* CF)
   constexpr int NT = P::T * CF;           // threads
   constexpr long long Nh = N / 2;
   constexpr long long plane = (long long)N * N * Nh;
-  __shared__ float2 smem[N * CF];
-  // W table [column][|fx|] (row CF: kz = N/2 for the packed slot); it aliases
-  // the transform's exchange buffer, which is idle between radii
-  static_assert((CF + 1) * NF <= 2 * N * CF, "W table must fit in the exchange buffer");
+  constexpr int SH = ilog2(P::radices[0]);
+  constexpr int NROW = N + (N >> SH);     // padded exchange rows
+  __shared__ float2 smem[NROW * CF];
+  // W table [|fx|][column] then the kz = N/2 column [|fx|]; it aliases the
+  // transform's exchange buffer, which is idle between radii
+  static_assert((CF + 1) * NF <= 2 * NROW * CF, "W table must fit in the exchange buffer");
   float* wtab = reinterpret_cast<float*>(smem);
+  float* wpk = wtab + CF * NF;
   const int tid = threadIdx.x;
   const int cl = tid % CF, i = tid / CF;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -263,16 +268,23 @@ __global__ void __launch_bounds__(P::T* CF)
     m[e] = packed ? D0[(long long)kx * N + ky] : sp[(long long)kx * C];
   }
   const int fy2 = fsq(ky, N);
+  const int kz0 = kzb * C + sub * CF;
   TS* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + c;  // element x at tp[x*N*C]
-  const LdsPencil lds{smem + cl, CF};
+  const LdsColPad<SH> lds{smem + cl, CF};
   const int ntab = (has_packed ? CF + 1 : CF) * NF;
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
     // tabulate W(|k| R)/N^3 for this block's columns
     for (int q = tid; q < ntab; q += NT) {
-      const int col = q / NF, f = q % NF;
-      const int kzq = (col < CF) ? kzb * C + sub * CF + col : N / 2;
+      int f, kzq;
+      if (q < CF * NF) {
+        f = q / CF;
+        kzq = kz0 + q % CF;
+      } else {
+        f = q - CF * NF;
+        kzq = N / 2;
+      }
       wtab[q] = filter_coef(f * f + fy2 + kzq * kzq, dkR, invN3);
     }
     __syncthreads();
@@ -281,10 +293,11 @@ __global__ void __launch_bounds__(P::T* CF)
     for (int e = 0; e < E; ++e) {
       const int kx = i + e * T;
       const int fa = (kx <= N / 2) ? kx : N - kx;
+      const float w0 = wtab[fa * CF + cl];
       if (!packed) {
-        v[e] = cscale(m[e], wtab[cl * NF + fa]);
+        v[e] = cscale(m[e], w0);
       } else {
-        const float w0 = wtab[cl * NF + fa], wn = wtab[CF * NF + fa];
+        const float wn = wpk[fa];
         const float2 b = DN[(long long)kx * N + ky];
         v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
       }
@@ -317,7 +330,7 @@ __global__ void __launch_bounds__(256)
          const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
   constexpr int L = P::L, T = P::T, E = P::E;
   constexpr int ROWS = 256 / T;
-  __shared__ float2 smem[ROWS * L];
+  __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
   __shared__ double red[4][kFusedMaxRadii];
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
@@ -325,7 +338,7 @@ __global__ void __launch_bounds__(256)
   if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
   __syncthreads();
   const long long plane = nrows * L;
-  const LdsPencil lds{smem + rr * L, 1};
+  const LdsRow<L> lds{smem + rr * LdsRow<L>::STRIDE};
 #pragma unroll 1
   for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
     const long long row = base + rr;
@@ -528,10 +541,27 @@ static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, T
 }
 
 template <int N>
-static int fz_grid(long long nrows) {
-  constexpr int ROWS = 256 / Plan<N / 2>::T;
-  const long long need = (nrows + ROWS - 1) / ROWS;
-  return (int)(need < 2048 ? need : 2048);
+static int fz_grid(bool half, bool rb) {
+  // grid-stride kernel: one block per resident slot of the device (all blocks
+  // run in one wave of the grid), never more blocks than row groups
+  using P = Plan<N / 2>;
+  constexpr int C = Blk<N>::C;
+  constexpr int ROWS = 256 / P::T;
+  const long long need = ((long long)N * N + ROWS - 1) / ROWS;
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) {
+    if (half)
+      e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, true, __half2>, 256, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, false, __half2>, 256, 0);
+    else
+      e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, true, float2>, 256, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, false, float2>, 256, 0);
+  }
+  long long g = (e == hipSuccess && cus > 0 && per_cu > 0) ? (long long)cus * per_cu : 2048;
+  if (g > kFzMaxBlocks) g = kFzMaxBlocks;
+  return (int)(need < g ? need : g);
 }
 
 template <int N, class TS>
@@ -695,11 +725,11 @@ hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN,
   }
 }
 
-int fz_blocks(int N) {
+int fz_blocks(int N, bool half, bool rb) {
   switch (N) {
 #define X(NN) \
   case NN:    \
-    return fz_grid<NN>((long long)NN * NN);
+    return fz_grid<NN>(half, rb);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

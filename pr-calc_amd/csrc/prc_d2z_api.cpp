@@ -65,7 +65,7 @@ struct KernelTimer {
 };
 
 constexpr int kDefaultRadiiPerLaunch = 8;
-constexpr int kMaxZBlocks = 4096;  // largest grid of the fused z pass
+constexpr int kMaxZBlocks = kFzMaxBlocks;  // largest grid of the fused z pass
 constexpr int kMomentBlocks = 1024;
 
 bool supported_n(int N) {
@@ -461,7 +461,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   // memory budget check before allocating
   const int NS = p->Nscales;
-  c->nblk_fz = fz_blocks(p->N);
+  c->nblk_fz = fz_blocks(p->N, p->mode == PRC_D2Z_FUSED_MIXED, p->RecordBubbleSizes != 0);
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
   const size_t tstore = (p->mode == PRC_D2Z_FUSED_MIXED) ? sizeof(__half2) : sizeof(float2);
@@ -765,7 +765,12 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
   const int nr = std::min(c->p.Nscales, 8);
   std::vector<int> nbs;
   for (int nb = c->nbatch_cap; nb >= 1; nb = (nb > 1) ? nb / 2 : 0) nbs.push_back(nb);
-  const int grids[] = {512, 1024, 2048, 4096};
+  // the resident-capacity grid chosen at create, its double, and fixed sizes
+  const int g0 = fz_blocks(c->p.N, c->p.mode == PRC_D2Z_FUSED_MIXED, c->p.RecordBubbleSizes != 0);
+  std::vector<int> grids = {g0, 512, 1024, 2048, 4096};
+  if (2 * g0 <= kMaxZBlocks) grids.push_back(2 * g0);
+  std::sort(grids.begin(), grids.end());
+  grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
   hipEvent_t a = nullptr, b = nullptr;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);

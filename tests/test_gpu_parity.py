@@ -263,6 +263,46 @@ def test_c2_size_512(mode):
         assert tab[r, 2] == pytest.approx(np.sqrt(np.mean(ref ** 2)), rel=1e-5)
 
 
+def test_c3_size_1024():
+    """1024^3 / 2000 Mpc/h with 96 radii (the benchmark box).  The smallest and
+    largest radius' smoothed fields against the FP64 oracle
+    (scipy's threaded FFT); the fused zreion against the plain path's; the
+    mixed path's zreion and ionized fraction against the fused FP32 result."""
+    N = 1024
+    from prc_d2z import D2ZParams
+    mk = lambda mode: D2ZParams(N=N, BoxSize=2000.0, Nscales=96, Rmax=120.0, zeta=40.0, M_min=1.4706e8,
+                                mode=mode)
+    out = {}
+    for mode in (FUSED_FP32, PLAIN_FP32, 2):
+        announce(f"1024^3 mode={mode}")
+        with D2Z(mk(mode)) as d:
+            d.generate_delta(1)
+            d.run()
+            out[mode] = (d.zreion(), d.history())
+            if mode == FUSED_FP32:
+                delta = d.delta()
+                fields = {r: d.smoothed_field(r) for r in (0, 95)}
+    z32, h32 = out[FUSED_FP32]
+    assert np.all(z32 >= 0) and np.isfinite(z32).all()
+    assert np.max(np.abs(out[PLAIN_FP32][0] - z32)) <= 1e-5
+    dz = np.abs(out[2][0].astype(np.float64) - z32)
+    x32 = np.cumsum(h32.astype(np.int64)) / z32.size
+    x16 = np.cumsum(out[2][1].astype(np.int64)) / z32.size
+    print(f"1024^3 mixed vs fp32: max|dz| {dz.max():.3e} x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
+    assert dz.max() <= 0.02
+    assert np.max(np.abs(x32 - x16)) <= 1e-3
+    del out, dz
+    announce("1024^3 FP64 oracle, radii 0 and 95")
+    sc = oracle.setup_scalars(oracle_params(mk(FUSED_FP32)))
+    dhat = oracle.forward(delta, sc.DInit, workers=16)
+    kmag = oracle.kmag_grid(N, sc.box_mpc)
+    for r, g in fields.items():
+        ref = oracle.smooth(dhat, kmag, sc.R[r], workers=16)
+        err = np.max(np.abs(g - ref)) / np.sqrt(np.mean(ref ** 2))
+        print(f"1024^3 r={r} rel max err {err:.2e}")
+        assert err <= 5e-5
+
+
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, 2])
 def test_graph_replay_matches_eager(mode):
     """The captured run is replayed several times, with eager library work and
