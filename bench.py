@@ -67,9 +67,10 @@ def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:
     return table.get(name, 0.0) * cells
 
 
-def traffic_from_profiles(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
-    f = ROOT / "profiles" / f"traffic_{workload}.json"
+def traffic_from_profiles(kernel: str, workload: str, mode: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated), if any."""
+    f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
     if not f.exists():
         return None
     try:
@@ -153,7 +154,7 @@ def main():
     achieved = alg / avg_s / 1e9 if alg else None
     roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic_from_profiles(dom_name, args.workload),
+            "traffic": traffic_from_profiles(dom_name, args.workload, args.mode),
             "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": alg,
             "path_achieved": value * B_ALG_PATH / 1e9,
             "path_frac": value * B_ALG_PATH / 1e9 / HBM_PEAK_GBS}

@@ -94,6 +94,15 @@ __device__ __forceinline__ float filter_coef(int m, float dkR, float invN3) {
   return tophat(__builtin_amdgcn_sqrtf((float)m) * dkR) * invN3;
 }
 
+// The same in two steps for tables that reuse |k| over radii:
+// kmag_grid(m) = |k| / dk, filter_of_kmag(kmag_grid(m), dkR, invN3) ==
+// filter_coef(m, dkR, invN3) bit for bit.
+__device__ __forceinline__ float kmag_grid(int m) { return __builtin_amdgcn_sqrtf((float)m); }
+__device__ __forceinline__ float filter_of_kmag(float km, float dkR, float invN3) {
+  if (km == 0.0f) return 0.0f;
+  return tophat(km * dkR) * invN3;
+}
+
 // -------------------------------------- barrier: d2z() in double, stored float
 // z = A*(delta_R + B) - 1 with A float and B = sqrt(2 (smin^2 - s^2)) * erfizeta
 // in double (delta2zreion_d2z.C:206-216); separate multiply and subtract (no
@@ -101,6 +110,15 @@ __device__ __forceinline__ float filter_coef(int m, float dkR, float invN3) {
 __device__ __forceinline__ float barrier_z(float d, double A, double B) {
 #pragma clang fp contract(off)
   const double s = (double)d + B;
+  const double p = A * s;
+  return (float)(p - 1.0);
+}
+
+// barrier_z of d * sc for a power-of-two sc: (double)d * sc is exact, so the
+// FMA rounds once, as (double)(d * sc) + B does; sc = 1 gives barrier_z(d).
+__device__ __forceinline__ float barrier_z_scaled(float d, double sc, double A, double B) {
+#pragma clang fp contract(off)
+  const double s = __builtin_fma((double)d, sc, B);
   const double p = A * s;
   return (float)(p - 1.0);
 }

@@ -226,11 +226,12 @@ __global__ void k_filter(const float2* __restrict__ S, const float2* __restrict_
 // intermediate T_r = Tbuf + r_local * N*N*N/2 (T layout).
 //
 // W depends on kx only through |fx| (fx = kx or kx - N), so per radius the
-// block first tabulates W for its CF columns and the N/2 + 1 values of |fx|
-// in LDS (half the filter evaluations of a per-element loop), plus the
-// kz = N/2 column for the packed slot.  The table is laid out [|fx|][column]
-// so that the lanes of a wave (consecutive columns, then consecutive kx) read
-// consecutive words.
+// block first tabulates W in LDS for its CF columns and the N/2 + 1 values of
+// |fx| (half the filter evaluations of a per-element loop), plus the kz = N/2
+// column for the packed slot.  |k| of every table entry is tabulated once
+// before the radius loop.  The table is laid out [|fx|][column] so that the
+// lanes of a wave (consecutive columns, then consecutive kx) read consecutive
+// words.
 template <class P, int C, int CF, class TS>
 __global__ void __launch_bounds__(P::T* CF)
     k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
@@ -245,22 +246,23 @@ __global__ void __launch_bounds__(P::T* CF)
   constexpr int SH = ilog2(P::radices[0]);
   constexpr int NROW = N + (N >> SH);     // padded exchange rows
   __shared__ float2 smem[NROW * CF];
-  // W table [|fx|][column] then the kz = N/2 column [|fx|]; it aliases the
+  // W table [|fx|][CF] then the kz = N/2 column [|fx|]; it aliases the
   // transform's exchange buffer, which is idle between radii
   static_assert((CF + 1) * NF <= 2 * NROW * CF, "W table must fit in the exchange buffer");
   float* wtab = reinterpret_cast<float*>(smem);
   float* wpk = wtab + CF * NF;
+  __shared__ float ktab[(CF + 1) * NF];  // |k|/dk of the table entries, same order
   const int tid = threadIdx.x;
   const int cl = tid % CF, i = tid / CF;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   const int sub = (int)(tile % NSUB);
   const int ky = (int)((tile / NSUB) % N);
   const int kzb = (int)(tile / (NSUB * N));
-  const int c = sub * CF + cl;
-  const int kz = kzb * C + c;
+  const int kz0 = kzb * C + sub * CF;
+  const int kz = kz0 + cl;
   const bool packed = (kz == 0);
   const bool has_packed = (kzb == 0 && sub == 0);  // block-uniform
-  const float2* sp = S + ((long long)kzb * N + ky) * N * C + c;  // element kx at sp[kx*C]
+  const float2* sp = S + ((long long)kzb * N + ky) * N * C + sub * CF + cl;  // element kx at sp[kx*C]
   float2 m[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -268,25 +270,27 @@ __global__ void __launch_bounds__(P::T* CF)
     m[e] = packed ? D0[(long long)kx * N + ky] : sp[(long long)kx * C];
   }
   const int fy2 = fsq(ky, N);
-  const int kz0 = kzb * C + sub * CF;
-  TS* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + c;  // element x at tp[x*N*C]
+  TS* tp = Tbuf + (long long)kzb * N * N * C + (long long)ky * C + sub * CF + cl;  // element x at tp[x*N*C]
   const LdsColPad<SH> lds{smem + cl, CF};
   const int ntab = (has_packed ? CF + 1 : CF) * NF;
+  for (int q = tid; q < ntab; q += NT) {
+    int f, kzq;
+    if (q < CF * NF) {
+      f = q / CF;
+      kzq = kz0 + q % CF;
+    } else {
+      f = q - CF * NF;
+      kzq = N / 2;
+    }
+    ktab[q] = kmag_grid(f * f + fy2 + kzq * kzq);
+  }
+  __syncthreads();  // ktab complete
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
-    // tabulate W(|k| R)/N^3 for this block's columns
-    for (int q = tid; q < ntab; q += NT) {
-      int f, kzq;
-      if (q < CF * NF) {
-        f = q / CF;
-        kzq = kz0 + q % CF;
-      } else {
-        f = q - CF * NF;
-        kzq = N / 2;
-      }
-      wtab[q] = filter_coef(f * f + fy2 + kzq * kzq, dkR, invN3);
-    }
+    // tabulate W(|k| R)/N^3 for this block's columns (the exchange buffer is
+    // free: pencil_fft's last LDS access precedes its last barrier)
+    for (int q = tid; q < ntab; q += NT) wtab[q] = filter_of_kmag(ktab[q], dkR, invN3);
     __syncthreads();
     float2 v[E];
 #pragma unroll
@@ -339,6 +343,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const long long plane = nrows * L;
   const LdsRow<L> lds{smem + rr * LdsRow<L>::STRIDE};
+  const double sc = (double)iscale, sc2 = sc * sc;
 #pragma unroll 1
   for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
     const long long row = base + rr;
@@ -361,26 +366,31 @@ __global__ void __launch_bounds__(256)
       mirror_lanes<P>(v, m, i);
       c2r_pre<P>(v, m, i, tw_full);
       pencil_fft<P, +1>(v, i, tw_half, lds);
-#pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = cscale(v[e], iscale);  // power of two: exact
+      // the output scale iscale (a power of two) is applied inside the
+      // barrier's FMA and to the sum of squares, where it is exact
       const double A = rc[r0 + q].A, B = rc[r0 + q].B;
       const float R = rc[r0 + q].R;
       float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const float z0 = barrier_z(v[e].x, A, B);
-        const float z1 = barrier_z(v[e].y, A, B);
+        const float z0 = barrier_z_scaled(v[e].x, sc, A, B);
+        const float z1 = barrier_z_scaled(v[e].y, sc, A, B);
         acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
-        if (z0 > best[e].x) {
-          best[e].x = z0;
-          if (RB) bR0[e] = R;
-        }
-        if (z1 > best[e].y) {
-          best[e].y = z1;
-          if (RB) bR1[e] = R;
+        if constexpr (RB) {
+          if (z0 > best[e].x) {
+            best[e].x = z0;
+            bR0[e] = R;
+          }
+          if (z1 > best[e].y) {
+            best[e].y = z1;
+            bR1[e] = R;
+          }
+        } else {  // same result as the strict comparison (z is never NaN or -0)
+          best[e].x = fmaxf(best[e].x, z0);
+          best[e].y = fmaxf(best[e].y, z1);
         }
       }
-      const double wsum = wave_sum(valid ? (double)acc : 0.0);
+      const double wsum = wave_sum(valid ? (double)acc * sc2 : 0.0);
       if (lane == 0) red[w][q] += wsum;
     }
     if (valid) {

@@ -32,8 +32,10 @@ Nscales       {ns}
 """
 
 
-@pytest.mark.parametrize("mode", ["fused", "plain"])
+@pytest.mark.parametrize("mode", ["fused", "plain", "mixed"])
 def test_driver_end_to_end(tmp_path, mode):
+    # FP32 paths to the parity bar; the FP16-storage path to its own (max |dz| 0.02)
+    tol_z, tol_tau = (0.02, 2e-3) if mode == "mixed" else (2e-4, 1e-4)
     g = np.load(FIX, allow_pickle=False)
     N = int(g["N"])
     g["delta"].astype(np.float32).tofile(tmp_path / "delta")
@@ -46,7 +48,7 @@ def test_driver_end_to_end(tmp_path, mode):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     z = np.fromfile(tmp_path / "output" / "run.zreion", dtype=np.float32).reshape(N, N, N)
-    assert np.max(np.abs(z.astype(np.float64) - g["zreion"])) <= 2e-4
+    assert np.max(np.abs(z.astype(np.float64) - g["zreion"])) <= tol_z
     rows = re.findall(r"^\s+(\d+)\s+([\d.]+)\s+([\d.]+)\s+([\d.]+)$", r.stdout, re.M)
     assert len(rows) == int(g["Nscales"])
     np.testing.assert_allclose([float(x[2]) for x in rows], g["sigma"], atol=1e-3)
@@ -54,7 +56,7 @@ def test_driver_end_to_end(tmp_path, mode):
     tau = float(re.search(r"tau = ([\d.]+)", r.stdout).group(1))
     ref_tau = oracle.tau(oracle.history_z(), oracle.history_float(oracle.history_counts(g["zreion"]), N),
                          0.31, 0.049, 0.68)
-    assert tau == pytest.approx(float(ref_tau), rel=1e-4)
+    assert tau == pytest.approx(float(ref_tau), rel=tol_tau)
     used = (tmp_path / "param.d2z-usedvalues").read_text()
     assert "Nscales" in used and "DeltaFile" in used
 
