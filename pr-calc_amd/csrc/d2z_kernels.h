@@ -50,6 +50,9 @@ hipError_t launch_twiddles(float2* tw, int L, hipStream_t s);
 // per-stage twiddle tables of the length-N and length-N/2 plans
 int stage_twiddle_size(int N, bool half);
 hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t s);
+// per-stage twiddles of the fused x pass's plan
+int fx_twiddle_size(int N);
+hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s);
 // pencil passes (pass_kernels.hip)
 hipError_t launch_fwd_z(int N, const float* in, float2* out, float dinit, const float2* twh, const float2* twf,
                         hipStream_t s);

@@ -315,10 +315,27 @@ __global__ void __launch_bounds__(P::T* CF)
 }
 
 // columns per fused-x block: keep the block at <= 512 threads
+// Plan of the fused x pass.  It keeps a second register copy of its tile (the
+// unfiltered spectrum, reused for every radius), so at N = 1024 it uses 8
+// elements per thread (1024 = 8*8*4*4, 128 VGPRs) instead of the plain
+// passes' 16 (221 VGPRs): 16 waves per CU instead of 8.
+template <int N>
+struct FxPlan {
+  using P = Plan<N>;
+};
+template <>
+struct FxPlan<1024> {
+  using P = PlanT<1024, 8, 8, 8, 4, 4>;
+};
+
+// Columns per fused x block: at most 512 threads, except at N = 1024 where the
+// 8-element plan needs 8 columns x 128 threads to keep 64 B segments per row
+// (4 columns, 32 B segments, made the FP32 stores far slower).
 template <int N>
 struct FxCols {
+  using P = typename FxPlan<N>::P;
   static constexpr int C = Blk<N>::C;
-  static constexpr int CF = (Plan<N>::T * C <= 512) ? C : (Plan<N>::T * (C / 2) <= 512 ? C / 2 : C / 4);
+  static constexpr int CF = (N == 1024) ? 8 : ((P::T * C <= 512) ? C : (P::T * (C / 2) <= 512 ? C / 2 : C / 4));
 };
 
 // Last inverse pass (z, c2r) with the barrier fused: for each row the nb
@@ -477,6 +494,38 @@ int stage_twiddle_size(int N, bool half) {
   return half ? b : a;
 }
 
+int fx_twiddle_size(int N) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return plan_tw_size<typename FxPlan<NN>::P>();
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return -1;
+  }
+}
+
+template <int N>
+static hipError_t fx_twst_impl(float2* out, hipStream_t s) {
+  using P = typename FxPlan<N>::P;
+  constexpr int a = plan_tw_size<P>();
+  if (a > 0) hipLaunchKernelGGL((k_twst<P>), dim3((a + 255) / 256), dim3(256), 0, s, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s) {
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return fx_twst_impl<NN>(out, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t s) {
   switch (N) {
 #define X(NN) \
@@ -542,7 +591,7 @@ static hipError_t filter_impl(const float2* S, const float2* D0, const float2* D
 template <int N, class TS>
 static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, TS* Tbuf, float dk, float invN3,
                           const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
-  using P = Plan<N>;
+  using P = typename FxPlan<N>::P;
   constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
   const long long nblk = (long long)N * ((N / 2) / C) * (C / CF);
   hipLaunchKernelGGL((k_fx<P, C, CF, TS>), dim3((unsigned)nblk), dim3(P::T * CF), 0, s, S, D0, DN, Tbuf, dk, invN3,

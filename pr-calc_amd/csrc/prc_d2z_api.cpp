@@ -97,6 +97,7 @@ struct prc_d2z_ctx {
   float2* d_tw_full = nullptr;  // exp(+2 pi i k/N), k < N: real-transform glue
   float2* d_st_full = nullptr;  // per-stage twiddles of the length-N plan
   float2* d_st_half = nullptr;  // per-stage twiddles of the length-N/2 plan
+  float2* d_st_fx = nullptr;    // per-stage twiddles of the fused x pass's plan
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
   double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
   int nblk_fz = 0;
@@ -248,7 +249,7 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
-                               c->d_st_full, s);
+                               c->d_st_fx, s);
             }),
             "fused x pass");
   PRC_CHECK(c, timed(c, "inv_y", [&] {
@@ -494,6 +495,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_tw_full, sizeof(float2) * p->N);
   A_(c->d_st_full, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, false)));
   A_(c->d_st_half, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, true)));
+  A_(c->d_st_fx, sizeof(float2) * std::max(1, fx_twiddle_size(p->N)));
   A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
   A_(c->d_part, sizeof(double) * (size_t)NS * kMaxZBlocks);
   A_(c->d_sigma_grid, sizeof(float) * NS);
@@ -513,6 +515,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
                  p->zInit,  p->Rmin,   p->Rmax,   p->zeta, p->M_min, p->N,      p->Nscales};
   if ((e = launch_twiddles(c->d_tw_full, p->N, c->stream)) != hipSuccess ||
       (e = launch_stage_twiddles(p->N, c->d_st_full, c->d_st_half, c->stream)) != hipSuccess ||
+      (e = launch_fx_twiddles(p->N, c->d_st_fx, c->stream)) != hipSuccess ||
       (e = launch_setup(sp, c->d_integ, c->d_R, c->d_Rint, c->d_sc, c->d_rc, c->d_sigma, c->stream)) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, std::string("set-up kernels: ") + hipGetErrorString(e));
   c->rc.resize(NS);
@@ -700,7 +703,7 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
       const TileMap ym = tile_map(N, Axis::kYInPlace);
       hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
-                               c->d_st_full, c->stream);
+                               c->d_st_fx, c->stream);
       if (e == hipSuccess)
         e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, c->stream)
                  : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, c->stream);
