@@ -244,8 +244,20 @@ constexpr int plan_tw_size() {
   return plan_tw_offset<P, P::S>();
 }
 
+// Synchronisation between the LDS phases of a transform.  Pencils spread over
+// the waves of a block need a block barrier.  A pencil held by the lanes of
+// one wave only needs the compiler to keep its LDS accesses in program order:
+// the LDS performs the operations of one wave in order, so a wave-scope fence
+// (no instruction) plus a scheduling barrier suffices.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS view of one pencil: element a at base[a * sa]
 struct LdsPencil {
+  static constexpr bool kWaveLocal = false;
   float2* base;
   int sa;
   __device__ __forceinline__ float2& operator[](int a) const { return base[a * sa]; }
@@ -275,13 +287,18 @@ template <> struct RowSwizzle<1024> { static constexpr int SH = 4, MK = 31, PAD 
 // sit half a bank cycle apart.  Holds (L + L / 2^SH) * CF elements.
 template <int SH>
 struct LdsColPad {
+  static constexpr bool kWaveLocal = false;
   float2* base;  // column start
   int sa;        // CF
   __device__ __forceinline__ float2& operator[](int a) const { return base[(a + (a >> SH)) * sa]; }
 };
 
-template <int L>
+// With WAVE the pencils must be held by the lanes of one wave (T <= 64
+// consecutive lanes, 64 % T == 0) and the transforms synchronise per wave
+// instead of per block.
+template <int L, bool WAVE = false>
 struct LdsRow {
+  static constexpr bool kWaveLocal = WAVE;
   using Sw = RowSwizzle<L>;
   static constexpr int STRIDE = L + Sw::PAD;  // float2 per row
   float2* base;
@@ -343,17 +360,19 @@ __device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, V lds) {
 }
 
 // Full pencil transform: registers in natural layout -> registers in natural
-// layout.  All threads of the block must call it (it contains barriers); the
-// LDS region may be reused right after it returns.
+// layout.  All threads of the block (of the wave, for a wave-local view) must
+// call it (it contains barriers); the LDS region may be reused right after it
+// returns.
 template <class P, int DIR, int STAGE = 0, class V>
 __device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw, V lds) {
   if constexpr (STAGE == P::S - 1) {
     stockham_stage<P, STAGE, DIR, true>(v, i, tw, lds);
   } else {
+    static_assert(!V::kWaveLocal || (P::T <= 64 && 64 % P::T == 0), "wave-local pencil must fit in one wave");
     stockham_stage<P, STAGE, DIR, false>(v, i, tw, lds);
-    __syncthreads();
+    if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     lds_gather<P>(v, i, lds);
-    __syncthreads();
+    if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     pencil_fft<P, DIR, STAGE + 1>(v, i, tw, lds);
   }
 }

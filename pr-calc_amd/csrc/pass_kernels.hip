@@ -19,6 +19,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "d2z_device.h"
 #include "d2z_kernels.h"
 #include "fft_core.h"
@@ -359,7 +361,12 @@ __global__ void __launch_bounds__(256)
   if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
   __syncthreads();
   const long long plane = nrows * L;
-  const LdsRow<L> lds{smem + rr * LdsRow<L>::STRIDE};
+  // FP16 storage: the pass is issue-bound and each wave synchronises its own
+  // pencil.  FP32 storage: block barriers keep the 4 waves' loads (adjacent
+  // 128 B segments of each kz block) together in time, which measured 8%
+  // faster than letting the waves drift apart.
+  using Lds = LdsRow<L, std::is_same<TS, __half2>::value>;
+  const Lds lds{smem + rr * Lds::STRIDE};
   const double sc = (double)iscale, sc2 = sc * sc;
 #pragma unroll 1
   for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
