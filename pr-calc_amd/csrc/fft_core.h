@@ -315,7 +315,7 @@ struct LdsRow {
 // the plan's per-stage twiddle table.
 // If LAST, outputs stay in registers (natural order, v[e] = y[i + e*T]);
 // otherwise they are written to LDS at their Stockham positions.
-template <class P, int STAGE, int DIR, bool LAST, class V>
+template <class P, int STAGE, int DIR, bool LAST, class V, bool TWT = false>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const float2* __restrict__ tw,
                                                V lds) {
   constexpr int L = P::L, E = P::E, T = P::T;
@@ -333,10 +333,11 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[P::E], int i, const f
     const int b = i + m * T;
     const int k = (p == 1) ? 0 : (b % p);
     if constexpr (p > 1) {
-      const float2* wk = tw + TWO + k * R;
+      // TWT: the stage's table is stored transposed, entry (k, r) at r*p + k
+      const float2* wk = TWT ? tw + TWO + k : tw + TWO + k * R;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = wk[r];
+        float2 w = TWT ? wk[r * p] : wk[r];
         if (DIR < 0) w.y = -w.y;
         u[r] = cmul(u[r], w);
       }
@@ -363,17 +364,36 @@ __device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, V lds) {
 // layout.  All threads of the block (of the wave, for a wave-local view) must
 // call it (it contains barriers); the LDS region may be reused right after it
 // returns.
-template <class P, int DIR, int STAGE = 0, class V>
+template <class P, int DIR, int STAGE = 0, class V, bool TWT = false>
 __device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw, V lds) {
   if constexpr (STAGE == P::S - 1) {
-    stockham_stage<P, STAGE, DIR, true>(v, i, tw, lds);
+    stockham_stage<P, STAGE, DIR, true, V, TWT>(v, i, tw, lds);
   } else {
     static_assert(!V::kWaveLocal || (P::T <= 64 && 64 % P::T == 0), "wave-local pencil must fit in one wave");
-    stockham_stage<P, STAGE, DIR, false>(v, i, tw, lds);
+    stockham_stage<P, STAGE, DIR, false, V, TWT>(v, i, tw, lds);
     if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     lds_gather<P>(v, i, lds);
     if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
-    pencil_fft<P, DIR, STAGE + 1>(v, i, tw, lds);
+    pencil_fft<P, DIR, STAGE + 1, V, TWT>(v, i, tw, lds);
+  }
+}
+
+// Copy plan P's per-stage twiddle table src ([k][r] per stage) to dst with
+// every stage transposed ([r][k]), threads t = 0..nt-1 of the block.
+template <class P, int STAGE = 0>
+__device__ __forceinline__ void copy_twiddles_transposed(float2* dst, const float2* __restrict__ src, int t,
+                                                         int nt) {
+  if constexpr (STAGE < P::S) {
+    constexpr int p = plan_p<P, STAGE>();
+    constexpr int R = P::radices[STAGE];
+    constexpr int TWO = plan_tw_offset<P, STAGE>();
+    if constexpr (p > 1) {
+      for (int q = t; q < p * R; q += nt) {
+        const int k = q / R, r = q % R;
+        dst[TWO + r * p + k] = src[TWO + q];
+      }
+    }
+    copy_twiddles_transposed<P, STAGE + 1>(dst, src, t, nt);
   }
 }
 

@@ -355,10 +355,17 @@ __global__ void __launch_bounds__(256)
   constexpr int ROWS = 256 / T;
   __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
   __shared__ double red[4][kFusedMaxRadii];
+  // twiddles in LDS (the blocks are persistent: copied once), the plan's
+  // stages transposed so that lanes with consecutive k read consecutive words
+  constexpr int NTW = plan_tw_size<P>() > 0 ? plan_tw_size<P>() : 1;
+  __shared__ float2 s_twh[NTW];
+  __shared__ float2 s_twf[L];
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
   const int w = t >> 6, lane = t & 63;
   if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
+  copy_twiddles_transposed<P>(s_twh, tw_half, t, 256);
+  for (int q = t; q < L; q += 256) s_twf[q] = tw_full[q];
   __syncthreads();
   const long long plane = nrows * L;
   // FP16 storage: the pass is issue-bound and each wave synchronises its own
@@ -387,9 +394,12 @@ __global__ void __launch_bounds__(256)
       float2 v[E], m[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+      // read per radius from LDS rather than held in registers across radii
+      int zero = 0;
+      asm volatile("" : "+s"(zero));
       mirror_lanes<P>(v, m, i);
-      c2r_pre<P>(v, m, i, tw_full);
-      pencil_fft<P, +1>(v, i, tw_half, lds);
+      c2r_pre<P>(v, m, i, s_twf + zero);
+      pencil_fft<P, +1, 0, Lds, true>(v, i, s_twh + zero, lds);
       // the output scale iscale (a power of two) is applied inside the
       // barrier's FMA and to the sum of squares, where it is exact
       const double A = rc[r0 + q].A, B = rc[r0 + q].B;
