@@ -145,13 +145,18 @@ def main():
     kernels = kt["kernels"]
     dom = max(kernels.items(), key=lambda kv: kv[1][0])
     dom_name, (dom_ms, dom_n) = dom
-    nb = min(8, NS) if args.mode != "plain" else 1
-    if args.radii_per_launch:
-        nb = args.radii_per_launch
+    # radii per fused launch as the library runs them (it may lower the
+    # request to fit the intermediates in HBM)
+    nb = d.launch_config()["radii_per_launch"] if args.mode != "plain" else 1
     store = 2 if args.mode == "mixed" else 4
-    alg = kernel_alg_bytes(dom_name, N, nb, store)
-    avg_s = dom_ms / dom_n * 1e-3
-    achieved = alg / avg_s / 1e9 if alg else None
+    # algorithmic bytes of the dominant kernel over the step (the last fused
+    # launch may carry fewer radii), per launch on average
+    if args.mode == "plain" or dom_name not in ("fx_filter", "inv_y", "fz_barrier"):
+        alg_step = kernel_alg_bytes(dom_name, N, 1, store) * dom_n
+    else:
+        alg_step = sum(kernel_alg_bytes(dom_name, N, min(nb, NS - r0), store) for r0 in range(0, NS, nb))
+    alg = alg_step / dom_n
+    achieved = alg_step / (dom_ms * 1e-3) / 1e9 if alg_step else None
     roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic_from_profiles(dom_name, args.workload, args.mode),

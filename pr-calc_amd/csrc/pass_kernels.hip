@@ -318,9 +318,10 @@ __global__ void __launch_bounds__(P::T* CF)
 
 // columns per fused-x block: keep the block at <= 512 threads
 // Plan of the fused x pass.  It keeps a second register copy of its tile (the
-// unfiltered spectrum, reused for every radius), so at N = 1024 it uses 8
-// elements per thread (1024 = 8*8*4*4, 128 VGPRs) instead of the plain
-// passes' 16 (221 VGPRs): 16 waves per CU instead of 8.
+// unfiltered spectrum, reused for every radius), so at N = 1024 and 2048 it
+// uses 8 elements per thread (1024 = 8*8*4*4, 2048 = 8*8*8*4; 128 VGPRs)
+// instead of the plain passes' 16 (221-225 VGPRs): 16 waves per CU instead
+// of 8.
 template <int N>
 struct FxPlan {
   using P = Plan<N>;
@@ -329,15 +330,22 @@ template <>
 struct FxPlan<1024> {
   using P = PlanT<1024, 8, 8, 8, 4, 4>;
 };
+template <>
+struct FxPlan<2048> {
+  using P = PlanT<2048, 8, 8, 8, 8, 4>;
+};
 
-// Columns per fused x block: at most 512 threads, except at N = 1024 where the
-// 8-element plan needs 8 columns x 128 threads to keep 64 B segments per row
-// (4 columns, 32 B segments, made the FP32 stores far slower).
+// Columns per fused x block: at most 512 threads, except with the 8-element
+// plans, which take 1024 threads: 8 columns x 128 at N = 1024 (64 B segments
+// per row; 4 columns, 32 B segments, made the FP32 stores far slower) and
+// 4 columns x 256 at N = 2048 (C = 8 there).
 template <int N>
 struct FxCols {
   using P = typename FxPlan<N>::P;
   static constexpr int C = Blk<N>::C;
-  static constexpr int CF = (N == 1024) ? 8 : ((P::T * C <= 512) ? C : (P::T * (C / 2) <= 512 ? C / 2 : C / 4));
+  static constexpr int CF = (N == 1024)   ? 8
+                            : (N == 2048) ? 4
+                                          : ((P::T * C <= 512) ? C : (P::T * (C / 2) <= 512 ? C / 2 : C / 4));
 };
 
 // Last inverse pass (z, c2r) with the barrier fused: for each row the nb
@@ -368,11 +376,12 @@ __global__ void __launch_bounds__(256)
   for (int q = t; q < L; q += 256) s_twf[q] = tw_full[q];
   __syncthreads();
   const long long plane = nrows * L;
-  // FP16 storage: the pass is issue-bound and each wave synchronises its own
-  // pencil.  FP32 storage: block barriers keep the 4 waves' loads (adjacent
-  // 128 B segments of each kz block) together in time, which measured 8%
-  // faster than letting the waves drift apart.
-  using Lds = LdsRow<L, std::is_same<TS, __half2>::value>;
+  // FP16 storage with 8-element pencils (N <= 1024): the pass is issue-bound
+  // and each wave synchronises its own pencil.  Otherwise block barriers keep
+  // the waves' loads (adjacent segments of each kz block) together in time:
+  // wave-level sync measured 8% slower for FP32 at N = 1024 and 40% slower
+  // for FP16 at N = 2048.
+  using Lds = LdsRow<L, std::is_same<TS, __half2>::value && (E <= 8)>;
   const Lds lds{smem + rr * Lds::STRIDE};
   const double sc = (double)iscale, sc2 = sc * sc;
 #pragma unroll 1

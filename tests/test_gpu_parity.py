@@ -303,6 +303,44 @@ def test_c3_size_1024():
         assert err <= 5e-5
 
 
+def _chunked_max_abs_diff(a, b, planes=16):
+    """max |a - b| over the leading axis in slabs (bounded host temporaries)."""
+    return max(float(np.max(np.abs(a[k:k + planes].astype(np.float64) - b[k:k + planes])))
+               for k in range(0, a.shape[0], planes))
+
+
+def test_c5_size_2048():
+    """2048^3 / 4000 Mpc/h (the C5 box) on its first 4 of 128 radii: the fused
+    FP32 path (8-element fused x plan, 16-element z rows) against the plain
+    path, whose kernels share only the FFT core; the mixed path against fused
+    FP32; zreion floor and the history total."""
+    N = 2048
+    from prc_d2z import D2ZParams
+    r0 = 4000.0 / N  # one cell, Mpc/h: radii 0..3 of the 128 from one cell to 150 Mpc/h
+    mk = lambda mode: D2ZParams(N=N, BoxSize=4000.0, Nscales=4, Rmax=r0 * (150.0 / r0) ** (3 / 127),
+                                zeta=40.0, M_min=1.4706e8, mode=mode)
+    out = {}
+    for mode in (FUSED_FP32, PLAIN_FP32, 2):
+        announce(f"2048^3 mode={mode}")
+        with D2Z(mk(mode)) as d:
+            d.generate_delta(1)
+            d.run()
+            out[mode] = (d.zreion(), d.history())
+    z32, h32 = out[FUSED_FP32]
+    assert min(float(z32[k:k + 64].min()) for k in range(0, N, 64)) >= 0.0
+    # every 4th x plane (2.1e9 cells) keeps the host comparison to seconds
+    d_plain = _chunked_max_abs_diff(out[PLAIN_FP32][0][::4], z32[::4])
+    d_mixed = _chunked_max_abs_diff(out[2][0][::4], z32[::4])
+    x32 = np.cumsum(h32.astype(np.int64)) / z32.size
+    x16 = np.cumsum(out[2][1].astype(np.int64)) / z32.size
+    print(f"2048^3 fused vs plain max|dz| {d_plain:.2e}; mixed vs fused max|dz| {d_mixed:.2e}, "
+          f"x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
+    assert d_plain <= 1e-5
+    assert d_mixed <= 0.02
+    assert np.max(np.abs(x32 - x16)) <= 1e-3
+    assert int(h32.sum()) <= z32.size
+
+
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, 2])
 def test_graph_replay_matches_eager(mode):
     """The captured run is replayed several times, with eager library work and
