@@ -50,6 +50,12 @@ hipError_t launch_twiddles(float2* tw, int L, hipStream_t s);
 // per-stage twiddle tables of the length-N and length-N/2 plans
 int stage_twiddle_size(int N, bool half);
 hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t s);
+// matrix-core fused x pass (mixed path, N = 1024); tw_full_plan is the
+// per-stage twiddle table of the length-N plan
+bool fx_mfma_supported(int N, bool half);
+hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
+                          float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
+                          hipStream_t s);
 // per-stage twiddles of the fused x pass's plan
 int fx_twiddle_size(int N);
 hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s);

@@ -248,6 +248,9 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   TileMap ym = tile_map(N, Axis::kYInPlace);
   ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
+              if (fx_mfma_supported(N, half))
+                return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
+                                      c->d_st_full, s);
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
                                c->d_st_fx, s);
             }),
@@ -702,8 +705,11 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       // the real output goes behind the first intermediate (FP32 sized region)
       float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
       const TileMap ym = tile_map(N, Axis::kYInPlace);
-      hipError_t e = launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
-                               c->d_st_fx, c->stream);
+      hipError_t e = fx_mfma_supported(N, half)
+                         ? launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
+                                          c->d_st_full, c->stream)
+                         : launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
+                                     c->d_st_fx, c->stream);
       if (e == hipSuccess)
         e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, c->stream)
                  : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, c->stream);
