@@ -6,7 +6,9 @@ of the density field, then for every radius the filter, c2r, barrier and
 running maximum (the work of c++/src/main_d2z.C:57-71 in
 marcelo-alvarez/pr-calc).  Default workload: 1024^3 cells, 2000 Mpc/h box,
 96 radii from one cell to 120 Mpc/h, zeta 40, M_min 1e8 Msun/h, input a
-synthetic Gaussian random field generated on the GPU (seed 1).
+synthetic Gaussian random field generated on the GPU (seed 1).  Default path:
+the batched mixed-precision fused path; the plain FP32 path is measured beside
+it on the same box (plain_fp32, speedup_vs_plain_fp32).
 
 Prints ONE JSON line.  Extra objects:
   roofline      dominant kernel: algorithmic bytes per launch / average launch
@@ -43,6 +45,9 @@ WORKLOADS = {
 # distinct seeds, 48 radii each from one cell to 50 Mpc/h
 MULTIBOX = [(128, 200.0), (192, 300.0), (256, 400.0)]
 MODES = {"plain": 0, "fused": 1, "mixed": 2}
+METRIC = "cell\u00b7radius updates/s at 1024\u00b3\u00d796R: batched mixed vs plain FP32; HBM GB/s"
+DTYPES = {"plain": "f32", "fused": "f32",
+          "mixed": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector; f64 barrier"}
 
 
 def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:
@@ -102,7 +107,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="fused", choices=list(MODES))
+    ap.add_argument("--mode", default="mixed", choices=list(MODES),
+                    help="mixed (default): fused path with FP16 intermediates and matrix-core radix-16 "
+                         "stages; fused: fused path in FP32; plain: one radius per pass in FP32")
+    ap.add_argument("--no-plain-compare", action="store_true",
+                    help="skip the plain FP32 path measured beside the fused paths")
     ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS) + ["c4_multibox"])
     ap.add_argument("--radii-per-launch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -176,6 +185,9 @@ def main():
     if args.history:
         h = d.history()
         print(f"history: {int(h.sum())} cells counted", file=sys.stderr)
+    plain = None
+    if args.mode != "plain" and not args.no_plain_compare:
+        plain = plain_fp32_rate(lp, steps=min(args.steps, 2))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -187,10 +199,10 @@ def main():
     d.close()
 
     out = {
-        "metric": "cell*radius updates/s at 1024^3 x 96 radii (delta2zreion)",
+        "metric": METRIC,
         "value": value, "unit": "cell*radius/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.mode != "mixed" else "f32/f16",
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[args.mode],
         "data": "synthetic",
         "config": {"workload": f"{N}^3 box {box:g} Mpc/h, {NS} radii 1 cell..{rmax:g} Mpc/h, "
                                f"zeta 40, Mmin 1e8 Msun/h, GRF seed 1", "mode": args.mode,
@@ -198,9 +210,29 @@ def main():
         "graph_ms_per_step": graph_ms,
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
+        "plain_fp32": plain,
+        "speedup_vs_plain_fp32": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def plain_fp32_rate(lp, steps: int) -> dict:
+    """The plain FP32 path (one radius per pass, no fusion) on the same box
+    (same generated field): the comparison the metric names."""
+    from prc_d2z import D2Z
+    N, NS = lp.N, lp.Nscales
+    d = D2Z(lp.copy(mode=MODES["plain"], radii_per_launch=0))
+    d.generate_delta(seed=1)
+    d.run()
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.run_async()
+    d.sync()
+    el = time.perf_counter() - t0
+    d.close()
+    return {"value": N ** 3 * NS * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps}
 
 
 def multibox(args):
