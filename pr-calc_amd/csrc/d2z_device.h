@@ -94,6 +94,21 @@ __device__ __forceinline__ float filter_coef(int m, float dkR, float invN3) {
   return tophat(__builtin_amdgcn_sqrtf((float)m) * dkR) * invN3;
 }
 
+// W(|k| R) / N^3 from the interpolation table (entries c0..c3 per interval of
+// length 1/invh in x = |k| R) for |k| = km dk; dkRh = dk R invh.  M = 0 or
+// km = 0 fall back to the exact value / the zeroed mean mode.
+__device__ __forceinline__ float filter_from_table(float km, float dkR, float dkRh, float invN3,
+                                                   const float4* tab, int M) {
+  if (km == 0.0f) return 0.0f;
+  if (M == 0) return tophat(km * dkR) * invN3;
+  const float u = km * dkRh;
+  int i = (int)u;
+  i = i < M - 1 ? i : M - 1;
+  const float t = u - (float)i;
+  const float4 c = tab[i];
+  return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x) * invN3;
+}
+
 // The same in two steps for tables that reuse |k| over radii:
 // kmag_grid(m) = |k| / dk, filter_of_kmag(kmag_grid(m), dkR, invN3) ==
 // filter_coef(m, dkR, invN3) bit for bit.

@@ -43,6 +43,21 @@ struct TileMap {
   int nOuter, nColBlocks;
 };
 enum class Axis { kYInPlace, kXTtoS, kXStoT };
+
+// Cubic Hermite table of the top-hat window W(x) = 3 (sin x - x cos x) / x^3 on
+// [0, M h): interval i holds (c0, c1, c2, c3) with W(x_i + t h) = c0 + t (c1 +
+// t (c2 + t c3)), built from W and W' in FP64 at the nodes.  M = 0: no table
+// (the kernels evaluate W directly).
+struct WTable {
+  const float4* coef;
+  int M;
+  float invh;  // 1 / h, a power of two
+};
+constexpr int kWTableMax = 3584;  // intervals (56 KB of LDS)
+// intervals and spacing covering [0, xmax]: h = 1/8 if it fits, else 1/4,
+// else 1/2; 0 intervals if none does
+int wtable_intervals(double xmax, double* h);
+hipError_t launch_wtable(float4* coef, int M, double h, hipStream_t s);
 int block_width(int N);
 TileMap tile_map(int N, Axis axis);
 
@@ -55,7 +70,7 @@ hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t 
 bool fx_mfma_supported(int N, bool half);
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
-                          hipStream_t s);
+                          const WTable& wt, hipStream_t s);
 // per-stage twiddles of the fused x pass's plan
 int fx_twiddle_size(int N);
 hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s);
@@ -74,7 +89,8 @@ hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const
 hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, const float2* tw, hipStream_t s);
 // fused passes; Tbuf holds float2 (half = false) or __half2 (half = true)
 hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
-                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s);
+                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, const WTable& wt,
+                     hipStream_t s);
 // blocks of the fused z pass: the device's resident capacity for that kernel,
 // at most kFzMaxBlocks and at most one per 256-thread row group
 constexpr int kFzMaxBlocks = 4096;

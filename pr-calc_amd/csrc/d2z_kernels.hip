@@ -439,6 +439,56 @@ hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double
   return hipGetLastError();
 }
 
+// ------------------------------------------------- W(x) interpolation table
+// W and W' in FP64: Taylor series below x = 1/2 (terms to x^12, truncation
+// below 1e-16), closed forms above.
+__device__ double w_exact(double x) {
+  if (x < 0.5) {
+    const double x2 = x * x;
+    return 1.0 - x2 * (1.0 / 10 - x2 * (1.0 / 280 - x2 * (1.0 / 15120 - x2 * (1.0 / 1330560 - x2 * (1.0 / 172972800 -
+                                                                                                    x2 / 31135104000.0)))));
+  }
+  double sn, cs;
+  sincos(x, &sn, &cs);
+  return 3.0 * (sn - x * cs) / (x * x * x);
+}
+__device__ double w_deriv(double x) {
+  if (x < 0.5) {
+    const double x2 = x * x;
+    return x * (-1.0 / 5 + x2 * (1.0 / 70 - x2 * (1.0 / 2520 - x2 * (1.0 / 166320 - x2 / 17297280.0))));
+  }
+  double sn, cs;
+  sincos(x, &sn, &cs);
+  return 3.0 * (x * x * sn - 3.0 * sn + 3.0 * x * cs) / (x * x * x * x);
+}
+__global__ void k_wtable(float4* __restrict__ coef, int M, double h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const double x0 = i * h, x1 = (i + 1) * h;
+  const double w0 = w_exact(x0), w1 = w_exact(x1);
+  const double d0 = w_deriv(x0) * h, d1 = w_deriv(x1) * h;
+  coef[i] = make_float4((float)w0, (float)d0, (float)(3.0 * (w1 - w0) - 2.0 * d0 - d1),
+                        (float)(2.0 * (w0 - w1) + d0 + d1));
+}
+
+int wtable_intervals(double xmax, double* h) {
+  for (double hh : {0.125, 0.25, 0.5}) {
+    const double m = ceil(xmax / hh) + 1.0;
+    if (m <= kWTableMax) {
+      *h = hh;
+      return (int)m;
+    }
+  }
+  *h = 0.0;
+  return 0;
+}
+
+hipError_t launch_wtable(float4* coef, int M, double h, hipStream_t s) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_wtable, dim3((M + 255) / 256), dim3(256), 0, s, coef, M, h);
+  return hipGetLastError();
+}
+
 hipError_t launch_count_nonfinite(const float* x, long long n, int* flag, hipStream_t s) {
   hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
   if (e != hipSuccess) return e;

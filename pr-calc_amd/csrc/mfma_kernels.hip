@@ -109,13 +109,14 @@ __device__ __forceinline__ void store_d(uint32_t* buf, const float4_t& d0, const
 __global__ void __launch_bounds__(kThreads)
     k_fx_mfma(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
               __half2* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
-              const float2* __restrict__ tw_full) {
+              const float2* __restrict__ tw_full, const float4* __restrict__ wcoef, int wM, float winvh) {
   constexpr long long Nh = kN / 2;
   constexpr long long plane = (long long)kN * kN * Nh;
   __shared__ uint32_t bufX[kN * kCF];
   __shared__ uint32_t bufY[kN * kCF];
   __shared__ float wtab[(kCF + 1) * kNF];  // [|fx|][column], then the kz = N/2 column
   __shared__ float ktab[(kCF + 1) * kNF];  // |k|/dk of the same entries
+  extern __shared__ float4 wsh[];           // W(x) interpolation table (wM intervals)
   float* wpk = wtab + kCF * kNF;
 
   const int tid = threadIdx.x;
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     ktab[q] = kmag_grid(f * f + fy2 + kzq * kzq);
   }
+  for (int q = tid; q < wM; q += kThreads) wsh[q] = wcoef[q];
   // stage-C thread layout: column cl, butterflies i and i + 128
   const int cl = tid & 7, ic = tid >> 3;
   __half2* tp = Tbuf + (long long)kzb * kN * kN * kC + (long long)ky * kC + sub * kCF + cl;
@@ -168,7 +170,8 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
-    for (int q = tid; q < ntab; q += kThreads) wtab[q] = filter_of_kmag(ktab[q], dkR, invN3);
+    const float dkRh = dkR * winvh;
+    for (int q = tid; q < ntab; q += kThreads) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, invN3, wsh, wM);
     __syncthreads();
     // stage A: filter, radix 16 on the matrix cores, outputs to bufX
 #pragma unroll
@@ -234,11 +237,11 @@ bool fx_mfma_supported(int N, bool half) { return half && N == kN; }
 
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
-                          hipStream_t s) {
-  if (N != kN || nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+                          const WTable& wt, hipStream_t s) {
+  if (N != kN || nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax) return hipErrorInvalidValue;
   const long long nblk = (long long)kN * ((kN / 2) / kC) * kNSUB;
-  hipLaunchKernelGGL(k_fx_mfma, dim3((unsigned)nblk), dim3(kThreads), 0, s, S, D0, DN,
-                     static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan);
+  hipLaunchKernelGGL(k_fx_mfma, dim3((unsigned)nblk), dim3(kThreads), sizeof(float4) * wt.M, s, S, D0, DN,
+                     static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan, wt.coef, wt.M, wt.invh);
   return hipGetLastError();
 }
 

@@ -238,7 +238,7 @@ template <class P, int C, int CF, class TS>
 __global__ void __launch_bounds__(P::T* CF)
     k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
          TS* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
-         const float2* __restrict__ tw) {
+         const float2* __restrict__ tw, const float4* __restrict__ wcoef, int wM, float winvh) {
   constexpr int N = P::L, T = P::T, E = P::E;
   constexpr int NSUB = C / CF;
   constexpr int NF = N / 2 + 1;           // distinct |fx|
@@ -254,6 +254,7 @@ __global__ void __launch_bounds__(P::T* CF)
   float* wtab = reinterpret_cast<float*>(smem);
   float* wpk = wtab + CF * NF;
   __shared__ float ktab[(CF + 1) * NF];  // |k|/dk of the table entries, same order
+  extern __shared__ float4 wsh[];         // W(x) interpolation table (wM intervals)
   const int tid = threadIdx.x;
   const int cl = tid % CF, i = tid / CF;
   const long long tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -286,13 +287,15 @@ __global__ void __launch_bounds__(P::T* CF)
     }
     ktab[q] = kmag_grid(f * f + fy2 + kzq * kzq);
   }
-  __syncthreads();  // ktab complete
+  for (int q = tid; q < wM; q += NT) wsh[q] = wcoef[q];
+  __syncthreads();  // ktab and the W(x) table complete
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
+    const float dkRh = dkR * winvh;
     // tabulate W(|k| R)/N^3 for this block's columns (the exchange buffer is
     // free: pencil_fft's last LDS access precedes its last barrier)
-    for (int q = tid; q < ntab; q += NT) wtab[q] = filter_of_kmag(ktab[q], dkR, invN3);
+    for (int q = tid; q < ntab; q += NT) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, invN3, wsh, wM);
     __syncthreads();
     float2 v[E];
 #pragma unroll
@@ -616,12 +619,13 @@ static hipError_t filter_impl(const float2* S, const float2* D0, const float2* D
 
 template <int N, class TS>
 static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, TS* Tbuf, float dk, float invN3,
-                          const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
+                          const RadiusConst* rc, int r0, int nb, const float2* tw, const WTable& wt,
+                          hipStream_t s) {
   using P = typename FxPlan<N>::P;
   constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
   const long long nblk = (long long)N * ((N / 2) / C) * (C / CF);
-  hipLaunchKernelGGL((k_fx<P, C, CF, TS>), dim3((unsigned)nblk), dim3(P::T * CF), 0, s, S, D0, DN, Tbuf, dk, invN3,
-                     rc, r0, nb, tw);
+  hipLaunchKernelGGL((k_fx<P, C, CF, TS>), dim3((unsigned)nblk), dim3(P::T * CF), sizeof(float4) * wt.M, s, S, D0,
+                     DN, Tbuf, dk, invN3, rc, r0, nb, tw, wt.coef, wt.M, wt.invh);
   return hipGetLastError();
 }
 
@@ -796,13 +800,14 @@ hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2*
 }
 
 hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
-                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, hipStream_t s) {
-  if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+                     float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, const WTable& wt,
+                     hipStream_t s) {
+  if (nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax) return hipErrorInvalidValue;
   switch (N) {
-#define X(NN)                                                                                             \
-  case NN:                                                                                                \
-    return half ? fx_impl<NN, __half2>(S, D0, DN, static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw, s) \
-                : fx_impl<NN, float2>(S, D0, DN, static_cast<float2*>(Tbuf), dk, invN3, rc, r0, nb, tw, s);
+#define X(NN)                                                                                                  \
+  case NN:                                                                                                     \
+    return half ? fx_impl<NN, __half2>(S, D0, DN, static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw, wt, s) \
+                : fx_impl<NN, float2>(S, D0, DN, static_cast<float2*>(Tbuf), dk, invN3, rc, r0, nb, tw, wt, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

@@ -98,6 +98,8 @@ struct prc_d2z_ctx {
   float2* d_st_full = nullptr;  // per-stage twiddles of the length-N plan
   float2* d_st_half = nullptr;  // per-stage twiddles of the length-N/2 plan
   float2* d_st_fx = nullptr;    // per-stage twiddles of the fused x pass's plan
+  float4* d_wcoef = nullptr;    // W(x) interpolation table of the fused x pass
+  WTable wt{nullptr, 0, 0.0f};
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
   double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
   int nblk_fz = 0;
@@ -250,9 +252,9 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               if (fx_mfma_supported(N, half))
                 return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
-                                      c->d_st_full, s);
+                                      c->d_st_full, c->wt, s);
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
-                               c->d_st_fx, s);
+                               c->d_st_fx, c->wt, s);
             }),
             "fused x pass");
   PRC_CHECK(c, timed(c, "inv_y", [&] {
@@ -527,6 +529,23 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
           hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, std::string("set-up copy: ") + hipGetErrorString(e));
+  // W(x) table for the fused x pass, covering x = |k| R up to the largest
+  // radius and the corner of the k grid (with a small margin)
+  {
+    float rmax = 0.0f;
+    for (const RadiusConst& r : c->rc) rmax = std::max(rmax, r.R);
+    const double xmax = (double)host_dk(c) * rmax * std::sqrt(3.0) * (p->N / 2) * 1.001 + 1.0;
+    double h = 0.0;
+    const int M = wtable_intervals(xmax, &h);
+    if (M > 0) {
+      if ((e = c->alloc(&c->d_wcoef, sizeof(float4) * M)) != hipSuccess)
+        return bail(PRC_D2Z_ERR_OOM, "device allocation failed: W table");
+      if ((e = launch_wtable(c->d_wcoef, M, h, c->stream)) != hipSuccess ||
+          (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return bail(PRC_D2Z_ERR_HIP, std::string("W table: ") + hipGetErrorString(e));
+      c->wt = WTable{c->d_wcoef, M, (float)(1.0 / h)};
+    }
+  }
   *out = c;
   return PRC_D2Z_OK;
 }
@@ -707,9 +726,9 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       const TileMap ym = tile_map(N, Axis::kYInPlace);
       hipError_t e = fx_mfma_supported(N, half)
                          ? launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
-                                          c->d_st_full, c->stream)
+                                          c->d_st_full, c->wt, c->stream)
                          : launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
-                                     c->d_st_fx, c->stream);
+                                     c->d_st_fx, c->wt, c->stream);
       if (e == hipSuccess)
         e = half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, c->stream)
                  : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, c->stream);
