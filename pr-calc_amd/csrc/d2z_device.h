@@ -138,6 +138,13 @@ __device__ __forceinline__ float barrier_z_scaled(float d, double sc, double A, 
   return (float)(p - 1.0);
 }
 
+// FP32 form for the mixed-precision path: sc is a power of two, so each fmaf
+// rounds once; |error| <= (|z| + 1) 2^-23, four orders below that path's own
+// FP16 error
+__device__ __forceinline__ float barrier_z_scaled_f32(float d, float sc, float A, float B) {
+  return fmaf(A, fmaf(d, sc, B), -1.0f);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

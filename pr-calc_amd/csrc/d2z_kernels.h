@@ -71,6 +71,11 @@ bool fx_mfma_supported(int N, bool half);
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
                           const WTable& wt, hipStream_t s);
+// matrix-core fused z pass (mixed path, N = 1024); tw_full = exp(2 pi i m / N)
+bool fz_mfma_supported(int N, bool half);
+int fz_mfma_blocks(bool rb);
+hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                          double* partial, int nblk, float iscale, const float2* tw_full, hipStream_t s);
 // per-stage twiddles of the fused x pass's plan
 int fx_twiddle_size(int N);
 hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s);

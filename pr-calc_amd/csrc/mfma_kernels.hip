@@ -231,7 +231,245 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// ---------------------------------------------------------------- z pass
+// Last inverse pass (z, c2r) of the mixed path at N = 1024 with the barrier
+// fused, as k_fz (reference delta2zreion_d2z.C:92-146,206-216), for blocks of
+// 8 consecutive z rows (512 B contiguous per kz block in the T layout).  Per
+// radius: the raw FP16 rows are staged in LDS; stage A forms the complex
+// sequence Z[k] of the half-length c2r from X[k] and X[L-k] (FP32) and runs
+// radix 16 on the matrix cores; stage B (radix 16, p = 16) groups the 16
+// butterflies {k, k+16} x 8 rows that share k = b mod 16; stage C (radix 2,
+// p = 256) leaves row w in wave w, which evaluates d2z() in FP64, keeps the
+// maximum over the batch in registers and the per-radius sums of squares.
+constexpr int kLz = kN / 2;   // complex points per z row
+constexpr int kRowsZ = 8;     // rows per block
+// Padded layouts (bank model: conflict-free except stage B's writes, 2-way);
+// every access is a per-lane base plus a compile-time offset, except the
+// mirrored reads X[L-k]
+constexpr int kXStride = 524, kY1Stride = 580, kY2Stride = 516;
+__device__ __forceinline__ int zx_at(int row, int pos) {  // raw rows X
+  return row * kXStride + pos + 2 * (pos >> 7);
+}
+__device__ __forceinline__ int zy1_at(int row, int pos) {  // stage A output
+  return row * kY1Stride + pos + 2 * (pos >> 4) + (pos >> 7);
+}
+__device__ __forceinline__ int zy2_at(int row, int pos) {  // stage B output (aliases X)
+  return row * kY2Stride + pos + (pos >> 7) + (pos >> 8);
+}
+
+template <bool RB>
+__global__ void __launch_bounds__(512)
+    k_fz_mfma(const __half2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub,
+              const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
+              const float2* __restrict__ tw_full) {
+  constexpr long long nrows = (long long)kN * kN;
+  constexpr long long plane = nrows * kLz;
+  __shared__ uint32_t bufX[kRowsZ * kXStride];  // also holds stage B's output (8 x 516)
+  __shared__ uint32_t bufY[kRowsZ * kY1Stride];
+  static_assert(kY2Stride <= kXStride, "stage B output must fit in the raw buffer");
+  __shared__ float2 s_twf[kLz];      // exp(2 pi i k / N), k < N/2 (c2r glue)
+  __shared__ float2 s_tw2[kLz / 2];  // exp(2 pi i b / (N/2)) (stage C)
+  __shared__ double red[kRowsZ][kFusedMaxRadii];
+  // operand matrices in the A layout, per lane: stage A [h][lane], stage B
+  // [wave][t][h][lane] (k = 2 wave + t); read per radius rather than held in
+  // registers across radii
+  __shared__ half8_t s_gA[2][64];
+  __shared__ half8_t s_gB[kRowsZ][2][2][64];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int g = l >> 4, n = l & 15;
+  const int rowA = n & 7, sel = n >> 3;  // MFMA column n = (row, butterfly sel)
+  if (tid < kRowsZ * kFusedMaxRadii) red[tid / kFusedMaxRadii][tid % kFusedMaxRadii] = 0.0;
+  for (int q = tid; q < kLz; q += 512) s_twf[q] = tw_full[q];
+  for (int q = tid; q < kLz / 2; q += 512) s_tw2[q] = tw_full[2 * q];
+  if (w < 2) s_gA[w][l] = stage_matrix(0, 1, w, l);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    s_gB[w][t][0][l] = stage_matrix(2 * w + t, 16, 0, l);
+    s_gB[w][t][1][l] = stage_matrix(2 * w + t, 16, 1, l);
+  }
+  const double sc = (double)iscale, sc2 = sc * sc;
+  // raw-load mapping: each half wave one row: kz = 16 kzb + (t & 15) with
+  // kzb = ((t >> 4) & 1) + 2 (t >> 8) + 4 e, row (t >> 5) & 7
+  const int lrow = (tid >> 5) & 7, lkz = tid & 15, lkzb = ((tid >> 4) & 1) + 2 * (tid >> 8);
+  const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
+  // the raw rows of (group, radius) are loaded into registers one step ahead
+  // and stored to LDS at the start of the step, so the loads overlap the
+  // previous step's transforms
+  uint32_t pf[8];
+  const uint32_t lane_off = ((uint32_t)lkzb * (uint32_t)nrows + (uint32_t)lrow) * 16u + (uint32_t)lkz;
+  auto fetch = [&](long long gbase, int q) {
+    const uint32_t* tp = reinterpret_cast<const uint32_t*>(Tbuf + (long long)q * plane + gbase * 16);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      // uniform part (kz block 4e) on the scalar base, one per-lane offset
+      const uint32_t* te = tp + (size_t)(4 * e) * nrows * 16;
+      pf[e] = te[lane_off];
+    }
+  };
+  const long long stride = (long long)gridDim.x * kRowsZ;
+  if ((long long)blockIdx.x * kRowsZ < nrows) fetch((long long)blockIdx.x * kRowsZ, 0);
+  __syncthreads();
+#pragma unroll 1
+  for (long long base = (long long)blockIdx.x * kRowsZ; base < nrows; base += stride) {
+    float2 best[8];
+    float bR0[8], bR1[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      best[e] = make_float2(-INFINITY, -INFINITY);
+      bR0[e] = 0.f;
+      bR1[e] = 0.f;
+    }
+#pragma unroll 1
+    for (int q = 0; q < nb; ++q) {
+      int zo = 0;  // opaque zero: the matrix reads stay inside the radius loop
+      asm volatile("" : "+s"(zo));
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bufX[zx_at(lrow, 16 * (lkzb + 4 * e) + lkz)] = pf[e];
+      __syncthreads();
+      // stage A: butterflies b = 4w + 2t + sel of row rowA, inputs Z[b + 32 r]
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int b = 4 * w + 2 * t + sel;
+        float2 zz[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = b + 32 * (4 * g + j);
+          const float2 a = unpack_h2(bufX[zx_at(rowA, k)]);
+          if (k == 0) {
+            zz[j] = make_float2(a.x + a.y, a.x - a.y);
+          } else {
+            const float2 mk = unpack_h2(bufX[zx_at(rowA, kLz - k)]);
+            const float2 bc = make_float2(mk.x, -mk.y);
+            const float2 sm = cadd(a, bc), df = csub(a, bc);
+            const float2 wd = cmul(s_twf[k], df);
+            zz[j] = make_float2(sm.x - wd.y, sm.y + wd.x);
+          }
+        }
+        const half8_t bop = pack_b(zz);
+        const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gA[0][l + zo], bop, z4, 0, 0, 0);
+        const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gA[1][l + zo], bop, z4, 0, 0, 0);
+        const int p0 = 16 * b + g;
+        bufY[zy1_at(rowA, p0)] = pack_h2(d0[0], d0[1]);
+        bufY[zy1_at(rowA, p0 + 4)] = pack_h2(d0[2], d0[3]);
+        bufY[zy1_at(rowA, p0 + 8)] = pack_h2(d1[0], d1[1]);
+        bufY[zy1_at(rowA, p0 + 12)] = pack_h2(d1[2], d1[3]);
+      }
+      __syncthreads();
+      if (q + 1 < nb)
+        fetch(base, q + 1);
+      else if (base + stride < nrows)
+        fetch(base + stride, 0);
+      // stage B: k = 2w + t, butterflies b' = k + 16 sel of row rowA, inputs
+      // b' + 32 r; outputs to the raw buffer, free since stage A's barrier
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int kk = 2 * w + t, bp = kk + 16 * sel;
+        union {
+          uint32_t u[4];
+          half8_t h;
+        } cv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cv.u[j] = bufY[zy1_at(rowA, bp + 32 * (4 * g + j))];
+        const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gB[w][t][0][l + zo], cv.h, z4, 0, 0, 0);
+        const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gB[w][t][1][l + zo], cv.h, z4, 0, 0, 0);
+        const int p0 = sel * 256 + kk + 16 * g;
+        bufX[zy2_at(rowA, p0)] = pack_h2(d0[0], d0[1]);
+        bufX[zy2_at(rowA, p0 + 64)] = pack_h2(d0[2], d0[3]);
+        bufX[zy2_at(rowA, p0 + 128)] = pack_h2(d1[0], d1[1]);
+        bufX[zy2_at(rowA, p0 + 192)] = pack_h2(d1[2], d1[3]);
+      }
+      __syncthreads();
+      // stage C: row w, butterflies b'' = l + 64 m; outputs n = b'' and b'' + 256
+      float2 v[8];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int b2 = l + 64 * m;
+        const float2 u0 = unpack_h2(bufX[zy2_at(w, b2)]);
+        const float2 u1 = cmul(unpack_h2(bufX[zy2_at(w, b2 + 256)]), s_tw2[b2]);
+        v[m] = cadd(u0, u1);
+        v[m + 4] = csub(u0, u1);
+      }
+      __syncthreads();  // the raw buffer is reloaded next radius
+      const float A = (float)rc[r0 + q].A, B = (float)rc[r0 + q].B;
+      const float R = rc[r0 + q].R;
+      float acc = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float z0 = barrier_z_scaled_f32(v[e].x, iscale, A, B);
+        const float z1 = barrier_z_scaled_f32(v[e].y, iscale, A, B);
+        acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
+        if constexpr (RB) {
+          if (z0 > best[e].x) {
+            best[e].x = z0;
+            bR0[e] = R;
+          }
+          if (z1 > best[e].y) {
+            best[e].y = z1;
+            bR1[e] = R;
+          }
+        } else {
+          best[e].x = fmaxf(best[e].x, z0);
+          best[e].y = fmaxf(best[e].y, z1);
+        }
+      }
+      const double wsum = wave_sum((double)acc * sc2);
+      if (l == 0) red[w][q] += wsum;
+    }
+    // zreion (and rbubble) of row base + w, element pair n = l + 64 m (+ 256)
+    float2* zp = reinterpret_cast<float2*>(zre + (base + w) * kN);
+    float2* bp2 = RB ? reinterpret_cast<float2*>(rbub + (base + w) * kN) : nullptr;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int nn = l + 64 * (e & 3) + 256 * (e >> 2);
+      const float2 old = zp[nn];
+      const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
+      if (u0 || u1) {
+        zp[nn] = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
+        if (RB) {
+          const float2 ob = bp2[nn];
+          bp2[nn] = make_float2(u0 ? bR0[e] : ob.x, u1 ? bR1[e] : ob.y);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < nb) {
+    double s = 0.0;
+    for (int k = 0; k < kRowsZ; ++k) s += red[k][tid];
+    partial[(long long)(r0 + tid) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
 }  // namespace
+
+bool fz_mfma_supported(int N, bool half) { return half && N == kN; }
+
+int fz_mfma_blocks(bool rb) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz_mfma<true>, 512, 0)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz_mfma<false>, 512, 0);
+  long long g = (e == hipSuccess && cus > 0 && per_cu > 0) ? (long long)cus * per_cu : 1024;
+  if (g > kFzMaxBlocks) g = kFzMaxBlocks;
+  const long long groups = (long long)kN * kN / kRowsZ;
+  return (int)(g < groups ? g : groups);
+}
+
+hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                          double* partial, int nblk, float iscale, const float2* tw_full, hipStream_t s) {
+  if (N != kN || nb < 1 || nb > kFusedMaxRadii || nblk < 1 || nblk > kFzMaxBlocks) return hipErrorInvalidValue;
+  const __half2* T = static_cast<const __half2*>(Tbuf);
+  if (rbub)
+    hipLaunchKernelGGL(k_fz_mfma<true>, dim3(nblk), dim3(512), 0, s, T, zre, rbub, rc, r0, nb, partial, iscale,
+                       tw_full);
+  else
+    hipLaunchKernelGGL(k_fz_mfma<false>, dim3(nblk), dim3(512), 0, s, T, zre, rbub, rc, r0, nb, partial, iscale,
+                       tw_full);
+  return hipGetLastError();
+}
 
 bool fx_mfma_supported(int N, bool half) { return half && N == kN; }
 

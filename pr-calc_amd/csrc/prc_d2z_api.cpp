@@ -185,6 +185,12 @@ int forward(prc_d2z_ctx* c, float dinit) {
   return PRC_D2Z_OK;
 }
 
+// grid of the fused z pass: the resident capacity of the kernel the mode uses
+int default_fz_blocks(const prc_d2z_params& p) {
+  const bool half = p.mode == PRC_D2Z_FUSED_MIXED, rb = p.RecordBubbleSizes != 0;
+  return fz_mfma_supported(p.N, half) ? fz_mfma_blocks(rb) : fz_blocks(p.N, half, rb);
+}
+
 float host_dk(const prc_d2z_ctx* c) {
   // float dk = 2.*M_PI/BoxSize with BoxSize already divided by h (delta2zreion_d2z.C:41)
   const float box = c->p.BoxSize / c->p.h;
@@ -263,8 +269,12 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
             }),
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
-              return launch_fz(N, c->d_T, half, c->d_zre, c->p.RecordBubbleSizes ? c->d_rb : nullptr, c->d_rc, r0, nb,
-                               c->d_part, c->nblk_fz, ms.inv, c->d_st_half, c->d_tw_full, s);
+              float* rb = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
+              if (fz_mfma_supported(N, half))
+                return launch_fz_mfma(N, c->d_T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
+                                      c->d_tw_full, s);
+              return launch_fz(N, c->d_T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
+                               c->d_st_half, c->d_tw_full, s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
@@ -467,7 +477,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   // memory budget check before allocating
   const int NS = p->Nscales;
-  c->nblk_fz = fz_blocks(p->N, p->mode == PRC_D2Z_FUSED_MIXED, p->RecordBubbleSizes != 0);
+  c->nblk_fz = default_fz_blocks(*p);
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
   const size_t tstore = (p->mode == PRC_D2Z_FUSED_MIXED) ? sizeof(__half2) : sizeof(float2);
@@ -794,7 +804,7 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
   std::vector<int> nbs;
   for (int nb = c->nbatch_cap; nb >= 1; nb = (nb > 1) ? nb / 2 : 0) nbs.push_back(nb);
   // the resident-capacity grid chosen at create, its double, and fixed sizes
-  const int g0 = fz_blocks(c->p.N, c->p.mode == PRC_D2Z_FUSED_MIXED, c->p.RecordBubbleSizes != 0);
+  const int g0 = default_fz_blocks(c->p);
   std::vector<int> grids = {g0, 512, 1024, 2048, 4096};
   if (2 * g0 <= kMaxZBlocks) grids.push_back(2 * g0);
   std::sort(grids.begin(), grids.end());

@@ -78,3 +78,30 @@ def test_mixed_against_fp32_at_512():
           f"x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
     assert dz.max() <= 0.02
     assert np.max(np.abs(x32 - x16)) <= 1e-3
+
+
+def test_mixed_rbubble_1024():
+    """1024^3 with bubble radii recorded, first 8 radii: the mixed path's
+    zreion and rbubble against the fused FP32 path."""
+    N = 1024
+    out = {}
+    for mode in (FUSED_FP32, FUSED_MIXED):
+        lp = D2ZParams(N=N, BoxSize=2000.0, Nscales=8, Rmax=2000.0 / N * (120.0 / (2000.0 / N)) ** (7 / 95),
+                       zeta=40.0, M_min=1.4706e8, mode=mode, RecordBubbleSizes=1)
+        announce(f"1024^3 rbubble mode={mode}")
+        with D2Z(lp) as d:
+            d.generate_delta(1)
+            d.run()
+            out[mode] = (d.zreion(), d.rbubble())
+    (z32, rb32), (z16, rb16) = out[FUSED_FP32], out[FUSED_MIXED]
+    dz = np.abs(z16.astype(np.float64) - z32)
+    # neighbouring radii are 4.4% apart: where z(R) is flat over several radii
+    # the winning radius moves with errors far below the mixed path's (about
+    # 2% of cells at this size)
+    radii = np.unique(np.concatenate([rb32[rb32 > 0], rb16[rb16 > 0]]))
+    mism = np.mean(rb16 != rb32)
+    print(f"1024^3 rbubble: mixed vs fp32 max|dz| {dz.max():.2e}, rbubble mismatch fraction {mism:.2e}")
+    assert radii.size <= 8
+    assert dz.max() <= 0.02
+    assert np.all((rb16 == 0) == (z16 == 0))
+    assert mism <= 0.05
