@@ -54,9 +54,10 @@ struct WTable {
   float invh;  // 1 / h, a power of two
 };
 constexpr int kWTableMax = 3584;  // intervals (56 KB of LDS)
-// intervals and spacing covering [0, xmax]: h = 1/8 if it fits, else 1/4,
-// else 1/2; 0 intervals if none does
-int wtable_intervals(double xmax, double* h);
+// intervals and spacing covering [0, xmax]: h = 1/8 if it fits in
+// max_intervals, else 1/4, else 1/2; 0 intervals if none does.  Max |error|
+// of the interpolant (W(0) = 1): 9.6e-8 at h = 1/8, 9.0e-7 at 1/4, 1.4e-5 at 1/2
+int wtable_intervals(double xmax, int max_intervals, double* h);
 hipError_t launch_wtable(float4* coef, int M, double h, hipStream_t s);
 int block_width(int N);
 TileMap tile_map(int N, Axis axis);
@@ -70,12 +71,17 @@ hipError_t launch_stage_twiddles(int N, float2* full, float2* half, hipStream_t 
 bool fx_mfma_supported(int N, bool half);
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
-                          const WTable& wt, hipStream_t s);
+                          const WTable& wt, const void* mats, hipStream_t s);
 // matrix-core fused z pass (mixed path, N = 1024); tw_full = exp(2 pi i m / N)
 bool fz_mfma_supported(int N, bool half);
 int fz_mfma_blocks(bool rb);
 hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                          double* partial, int nblk, float iscale, const float2* tw_full, hipStream_t s);
+                          double* partial, int nblk, float iscale, const float2* tw_full, const void* mats,
+                          hipStream_t s);
+// FP16 operand matrices of the matrix-core passes' radix-16 stages, built
+// once (0 bytes when neither pass runs for N and the storage type)
+size_t mfma_matrix_bytes(int N, bool half);
+hipError_t launch_mfma_matrices(void* out, hipStream_t s);
 // per-stage twiddles of the fused x pass's plan
 int fx_twiddle_size(int N);
 hipError_t launch_fx_twiddles(int N, float2* out, hipStream_t s);

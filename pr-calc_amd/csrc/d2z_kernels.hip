@@ -471,10 +471,10 @@ __global__ void k_wtable(float4* __restrict__ coef, int M, double h) {
                         (float)(2.0 * (w0 - w1) + d0 + d1));
 }
 
-int wtable_intervals(double xmax, double* h) {
+int wtable_intervals(double xmax, int max_intervals, double* h) {
   for (double hh : {0.125, 0.25, 0.5}) {
     const double m = ceil(xmax / hh) + 1.0;
-    if (m <= kWTableMax) {
+    if (m <= max_intervals) {
       *h = hh;
       return (int)m;
     }

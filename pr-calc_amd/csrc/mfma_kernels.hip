@@ -106,10 +106,22 @@ __device__ __forceinline__ void store_d(uint32_t* buf, const float4_t& d0, const
   buf[xrow(row_q0 + 12 * qstride) * kCF + c] = pack_h2(d1[2], d1[3]);
 }
 
+// Operand table: set 0, 1 = stage (k = 0, p = 1) blocks h = 0, 1; set
+// 2 + 2 k + h = stage (k, p = 16) block h; [set][lane], built once at set-up
+constexpr int kMatSets = 2 + 2 * 16;
+__device__ __forceinline__ int mat_set_a(int h) { return h; }
+__device__ __forceinline__ int mat_set_b(int k, int h) { return 2 + 2 * k + h; }
+
+__global__ void __launch_bounds__(64) k_mfma_mats(half8_t* __restrict__ out) {
+  const int set = blockIdx.x, l = threadIdx.x;
+  out[set * 64 + l] = set < 2 ? stage_matrix(0, 1, set, l) : stage_matrix((set - 2) >> 1, 16, (set - 2) & 1, l);
+}
+
 __global__ void __launch_bounds__(kThreads)
     k_fx_mfma(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
               __half2* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
-              const float2* __restrict__ tw_full, const float4* __restrict__ wcoef, int wM, float winvh) {
+              const float2* __restrict__ tw_full, const float4* __restrict__ wcoef, int wM, float winvh,
+              const half8_t* __restrict__ mats) {
   constexpr long long Nh = kN / 2;
   constexpr long long plane = (long long)kN * kN * Nh;
   __shared__ uint32_t bufX[kN * kCF];
@@ -133,8 +145,8 @@ __global__ void __launch_bounds__(kThreads)
   const bool packed = has_packed && c == 0;        // this lane's column is kz = 0
 
   // matrices: stage A (p = 1, k = 0) and stage B (p = 16, k = this wave)
-  const half8_t gA0 = stage_matrix(0, 1, 0, l), gA1 = stage_matrix(0, 1, 1, l);
-  const half8_t gB0 = stage_matrix(w, 16, 0, l), gB1 = stage_matrix(w, 16, 1, l);
+  const half8_t gA0 = mats[mat_set_a(0) * 64 + l], gA1 = mats[mat_set_a(1) * 64 + l];
+  const half8_t gB0 = mats[mat_set_b(w, 0) * 64 + l], gB1 = mats[mat_set_b(w, 1) * 64 + l];
 
   // the lane's 8 spectrum values: stage-A butterflies b = 4w + 2t + sel,
   // inputs r = 4g + j at kx = b + 64 r
@@ -261,7 +273,7 @@ template <bool RB>
 __global__ void __launch_bounds__(512)
     k_fz_mfma(const __half2* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub,
               const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
-              const float2* __restrict__ tw_full) {
+              const float2* __restrict__ tw_full, const half8_t* __restrict__ mats) {
   constexpr long long nrows = (long long)kN * kN;
   constexpr long long plane = nrows * kLz;
   __shared__ uint32_t bufX[kRowsZ * kXStride];  // also holds stage B's output (8 x 516)
@@ -282,11 +294,11 @@ __global__ void __launch_bounds__(512)
   if (tid < kRowsZ * kFusedMaxRadii) red[tid / kFusedMaxRadii][tid % kFusedMaxRadii] = 0.0;
   for (int q = tid; q < kLz; q += 512) s_twf[q] = tw_full[q];
   for (int q = tid; q < kLz / 2; q += 512) s_tw2[q] = tw_full[2 * q];
-  if (w < 2) s_gA[w][l] = stage_matrix(0, 1, w, l);
+  if (w < 2) s_gA[w][l] = mats[mat_set_a(w) * 64 + l];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    s_gB[w][t][0][l] = stage_matrix(2 * w + t, 16, 0, l);
-    s_gB[w][t][1][l] = stage_matrix(2 * w + t, 16, 1, l);
+    s_gB[w][t][0][l] = mats[mat_set_b(2 * w + t, 0) * 64 + l];
+    s_gB[w][t][1][l] = mats[mat_set_b(2 * w + t, 1) * 64 + l];
   }
   const double sc = (double)iscale, sc2 = sc * sc;
   // raw-load mapping: each half wave one row: kz = 16 kzb + (t & 15) with
@@ -459,15 +471,18 @@ int fz_mfma_blocks(bool rb) {
 }
 
 hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                          double* partial, int nblk, float iscale, const float2* tw_full, hipStream_t s) {
-  if (N != kN || nb < 1 || nb > kFusedMaxRadii || nblk < 1 || nblk > kFzMaxBlocks) return hipErrorInvalidValue;
+                          double* partial, int nblk, float iscale, const float2* tw_full, const void* mats,
+                          hipStream_t s) {
+  if (N != kN || nb < 1 || nb > kFusedMaxRadii || nblk < 1 || nblk > kFzMaxBlocks || !mats)
+    return hipErrorInvalidValue;
   const __half2* T = static_cast<const __half2*>(Tbuf);
+  const half8_t* M = static_cast<const half8_t*>(mats);
   if (rbub)
     hipLaunchKernelGGL(k_fz_mfma<true>, dim3(nblk), dim3(512), 0, s, T, zre, rbub, rc, r0, nb, partial, iscale,
-                       tw_full);
+                       tw_full, M);
   else
     hipLaunchKernelGGL(k_fz_mfma<false>, dim3(nblk), dim3(512), 0, s, T, zre, rbub, rc, r0, nb, partial, iscale,
-                       tw_full);
+                       tw_full, M);
   return hipGetLastError();
 }
 
@@ -475,11 +490,21 @@ bool fx_mfma_supported(int N, bool half) { return half && N == kN; }
 
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
-                          const WTable& wt, hipStream_t s) {
-  if (N != kN || nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax) return hipErrorInvalidValue;
+                          const WTable& wt, const void* mats, hipStream_t s) {
+  if (N != kN || nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax || !mats) return hipErrorInvalidValue;
   const long long nblk = (long long)kN * ((kN / 2) / kC) * kNSUB;
   hipLaunchKernelGGL(k_fx_mfma, dim3((unsigned)nblk), dim3(kThreads), sizeof(float4) * wt.M, s, S, D0, DN,
-                     static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan, wt.coef, wt.M, wt.invh);
+                     static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan, wt.coef, wt.M, wt.invh,
+                     static_cast<const half8_t*>(mats));
+  return hipGetLastError();
+}
+
+size_t mfma_matrix_bytes(int N, bool half) {
+  return (fx_mfma_supported(N, half) || fz_mfma_supported(N, half)) ? sizeof(half8_t) * kMatSets * 64 : 0;
+}
+
+hipError_t launch_mfma_matrices(void* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_mats, dim3(kMatSets), dim3(64), 0, s, static_cast<half8_t*>(out));
   return hipGetLastError();
 }
 

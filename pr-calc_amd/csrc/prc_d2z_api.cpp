@@ -99,6 +99,7 @@ struct prc_d2z_ctx {
   float2* d_st_half = nullptr;  // per-stage twiddles of the length-N/2 plan
   float2* d_st_fx = nullptr;    // per-stage twiddles of the fused x pass's plan
   float4* d_wcoef = nullptr;    // W(x) interpolation table of the fused x pass
+  void* d_mats = nullptr;       // operand matrices of the matrix-core passes
   WTable wt{nullptr, 0, 0.0f};
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
   double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
@@ -258,7 +259,7 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               if (fx_mfma_supported(N, half))
                 return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
-                                      c->d_st_full, c->wt, s);
+                                      c->d_st_full, c->wt, c->d_mats, s);
               return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
                                c->d_st_fx, c->wt, s);
             }),
@@ -272,7 +273,7 @@ int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
               float* rb = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
               if (fz_mfma_supported(N, half))
                 return launch_fz_mfma(N, c->d_T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
-                                      c->d_tw_full, s);
+                                      c->d_tw_full, c->d_mats, s);
               return launch_fz(N, c->d_T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                c->d_st_half, c->d_tw_full, s);
             }),
@@ -511,6 +512,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_st_full, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, false)));
   A_(c->d_st_half, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, true)));
   A_(c->d_st_fx, sizeof(float2) * std::max(1, fx_twiddle_size(p->N)));
+  const size_t mat_bytes = mfma_matrix_bytes(p->N, p->mode == PRC_D2Z_FUSED_MIXED);
+  if (mat_bytes) A_(c->d_mats, mat_bytes);
   A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
   A_(c->d_part, sizeof(double) * (size_t)NS * kMaxZBlocks);
   A_(c->d_sigma_grid, sizeof(float) * NS);
@@ -531,6 +534,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if ((e = launch_twiddles(c->d_tw_full, p->N, c->stream)) != hipSuccess ||
       (e = launch_stage_twiddles(p->N, c->d_st_full, c->d_st_half, c->stream)) != hipSuccess ||
       (e = launch_fx_twiddles(p->N, c->d_st_fx, c->stream)) != hipSuccess ||
+      (mat_bytes && (e = launch_mfma_matrices(c->d_mats, c->stream)) != hipSuccess) ||
       (e = launch_setup(sp, c->d_integ, c->d_R, c->d_Rint, c->d_sc, c->d_rc, c->d_sigma, c->stream)) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, std::string("set-up kernels: ") + hipGetErrorString(e));
   c->rc.resize(NS);
@@ -546,7 +550,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     for (const RadiusConst& r : c->rc) rmax = std::max(rmax, r.R);
     const double xmax = (double)host_dk(c) * rmax * std::sqrt(3.0) * (p->N / 2) * 1.001 + 1.0;
     double h = 0.0;
-    const int M = wtable_intervals(xmax, &h);
+    const int M = wtable_intervals(xmax, kWTableMax, &h);
     if (M > 0) {
       if ((e = c->alloc(&c->d_wcoef, sizeof(float4) * M)) != hipSuccess)
         return bail(PRC_D2Z_ERR_OOM, "device allocation failed: W table");
@@ -736,7 +740,7 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       const TileMap ym = tile_map(N, Axis::kYInPlace);
       hipError_t e = fx_mfma_supported(N, half)
                          ? launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
-                                          c->d_st_full, c->wt, c->stream)
+                                          c->d_st_full, c->wt, c->d_mats, c->stream)
                          : launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
                                      c->d_st_fx, c->wt, c->stream);
       if (e == hipSuccess)
