@@ -35,7 +35,7 @@ struct GrfSpectrum {
   double invN3;  // 1/N^3 of the inverse transform
 };
 
-constexpr int kFusedMaxRadii = 8;  // radii per fused batch
+constexpr int kFusedMaxRadii = 16;  // radii per fused batch
 
 // Strides of one tiled pass (complex elements); see pass_kernels.hip.
 struct TileMap {

@@ -64,7 +64,7 @@ struct KernelTimer {
   }
 };
 
-constexpr int kDefaultRadiiPerLaunch = 8;
+constexpr int kDefaultRadiiPerLaunch = kFusedMaxRadii;
 constexpr int kMaxZBlocks = kFzMaxBlocks;  // largest grid of the fused z pass
 constexpr int kMomentBlocks = 1024;
 
@@ -442,8 +442,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     g_create_error = "unknown mode";
     return PRC_D2Z_ERR_BAD_ARG;
   }
-  if (p->radii_per_launch < 0 || p->radii_per_launch > 8) {
-    g_create_error = "radii_per_launch must be in 0..8";
+  if (p->radii_per_launch < 0 || p->radii_per_launch > kFusedMaxRadii) {
+    g_create_error = "radii_per_launch must be in 0.." + std::to_string(kFusedMaxRadii);
     return PRC_D2Z_ERR_BAD_ARG;
   }
   int ndev = 0;
@@ -495,7 +495,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (have_info && need > freeb)
     return bail(PRC_D2Z_ERR_OOM, "device memory: need " + std::to_string(need >> 20) + " MiB, free " +
                                      std::to_string(freeb >> 20) + " MiB");
-  c->nbatch_cap = (p->mode == PRC_D2Z_PLAIN_FP32) ? 1 : (int)std::min<size_t>(8, c->tbytes / (tstore * c->nspec));
+  c->nbatch_cap = (p->mode == PRC_D2Z_PLAIN_FP32) ? 1 : (int)std::min<size_t>(kFusedMaxRadii, c->tbytes / (tstore * c->nspec));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
   hipError_t e = hipSuccess;

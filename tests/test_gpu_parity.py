@@ -82,9 +82,10 @@ def test_zreion_vs_oracle(N, mode):
 def test_fused_matches_plain_and_batching():
     N = 64
     outs = {}
-    for mode, rpl in [(PLAIN_FP32, 0), (FUSED_FP32, 1), (FUSED_FP32, 3), (FUSED_FP32, 8)]:
+    # 20 radii: batches of 16 + 4, 8 + 8 + 4, 3 x 6 + 2 and single radii
+    for mode, rpl in [(PLAIN_FP32, 0), (FUSED_FP32, 1), (FUSED_FP32, 3), (FUSED_FP32, 8), (FUSED_FP32, 16)]:
         announce(f"fused vs plain N={N} mode={mode} rpl={rpl}")
-        with D2Z(lib_params(N, nscales=10, mode=mode, rpl=rpl)) as d:
+        with D2Z(lib_params(N, nscales=20, mode=mode, rpl=rpl)) as d:
             d.set_delta(field(N))
             d.run()
             outs[(mode, rpl)] = d.zreion()
@@ -94,6 +95,7 @@ def test_fused_matches_plain_and_batching():
     # batching must not change the fused result at all
     assert np.array_equal(outs[(FUSED_FP32, 1)], outs[(FUSED_FP32, 8)])
     assert np.array_equal(outs[(FUSED_FP32, 3)], outs[(FUSED_FP32, 8)])
+    assert np.array_equal(outs[(FUSED_FP32, 16)], outs[(FUSED_FP32, 8)])
 
 
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32])
@@ -385,10 +387,10 @@ def test_autotune_and_launch_config(tmp_path):
         d.run()
         z_default = d.zreion()
         cfg0 = d.launch_config()
-        assert cfg0["radii_per_launch"] == 8 and cfg0["max_radii"] == 8
+        assert cfg0["radii_per_launch"] == 12 and cfg0["max_radii"] == 12  # all 12 radii in one batch
         cache = tmp_path / "tune.json"
         t = d.autotune(cache)
-        assert 1 <= t["radii_per_launch"] <= 8 and 512 <= t["z_blocks"] <= 4096 and not t["cached"]
+        assert 1 <= t["radii_per_launch"] <= 12 and 512 <= t["z_blocks"] <= 4096 and not t["cached"]
         d.run()
         assert np.array_equal(d.zreion(), z_default)   # launch shape never changes results
         assert d.autotune(cache)["cached"]
@@ -396,7 +398,7 @@ def test_autotune_and_launch_config(tmp_path):
         d.run()
         assert np.array_equal(d.zreion(), z_default)
         with pytest.raises(D2ZError):
-            d.set_launch_config(9, 1024)
+            d.set_launch_config(13, 1024)
 
 
 def test_run_many_matches_single_runs():
