@@ -47,7 +47,7 @@ MULTIBOX = [(128, 200.0), (192, 300.0), (256, 400.0)]
 MODES = {"plain": 0, "fused": 1, "mixed": 2}
 METRIC = "cell\u00b7radius updates/s at 1024\u00b3\u00d796R: batched mixed vs plain FP32; HBM GB/s"
 DTYPES = {"plain": "f32", "fused": "f32",
-          "mixed": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector; f64 barrier"}
+          "mixed": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector and barrier"}
 
 
 def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:

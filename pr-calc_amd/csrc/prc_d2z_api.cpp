@@ -248,37 +248,56 @@ MixedScales mixed_scales(int N) {
   return {ldexpf(1.0f, e1), ldexpf(1.0f, -e2), ldexpf(1.0f, e2 - e1)};
 }
 
-int fused_batch(prc_d2z_ctx* c, int r0, int nb) {
+// The x pass of one batch (filter + x inverse) into the intermediates T, on s
+int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
   const int N = c->N;
-  hipStream_t s = c->stream;
   const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
   const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
   const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
-  TileMap ym = tile_map(N, Axis::kYInPlace);
-  ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "fx_filter", [&] {
               if (fx_mfma_supported(N, half))
-                return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r0, nb,
+                return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, T, host_dk(c), invN3, c->d_rc, r0, nb,
                                       c->d_st_full, c->wt, c->d_mats, s);
-              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r0, nb,
+              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, T, half, host_dk(c), invN3, c->d_rc, r0, nb,
                                c->d_st_fx, c->wt, s);
             }),
             "fused x pass");
+  return PRC_D2Z_OK;
+}
+
+// The y pass and the z pass (c2r + barrier + max) of one batch over T, on s
+int fused_yz(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
+  const int N = c->N;
+  const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+  const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
+  TileMap ym = tile_map(N, Axis::kYInPlace);
+  ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
   PRC_CHECK(c, timed(c, "inv_y", [&] {
-              return half ? launch_tile_h(N, c->d_T, ym, ms.s2, c->d_st_full, s)
-                          : launch_tile(N, +1, c->d_T, c->d_T, ym, c->d_st_full, s);
+              return half ? launch_tile_h(N, T, ym, ms.s2, c->d_st_full, s)
+                          : launch_tile(N, +1, T, T, ym, c->d_st_full, s);
             }),
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
               float* rb = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
               if (fz_mfma_supported(N, half))
-                return launch_fz_mfma(N, c->d_T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
+                return launch_fz_mfma(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                       c->d_tw_full, c->d_mats, s);
-              return launch_fz(N, c->d_T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
+              return launch_fz(N, T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                c->d_st_half, c->d_tw_full, s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
+}
+
+// All fused batches, in order on one stream
+int fused_serial(prc_d2z_ctx* c) {
+  const int NS = c->p.Nscales;
+  int rc = PRC_D2Z_OK;
+  for (int r0 = 0; r0 < NS; r0 += c->nbatch) {
+    const int nb = std::min(c->nbatch, NS - r0);
+    if ((rc = fused_x(c, r0, nb, c->d_T, c->stream)) || (rc = fused_yz(c, r0, nb, c->d_T, c->stream))) return rc;
+  }
+  return rc;
 }
 
 // All launches of one run (forward transform through the last radius).
@@ -295,10 +314,7 @@ int enqueue_run(prc_d2z_ctx* c) {
       if ((rc = plain_radius(c, r, true))) return rc;
     PRC_CHECK(c, launch_moments_finish(c->d_mom, kMomentBlocks, NS, invN3, c->d_sigma_grid, 0, s), "sigma_grid");
   } else {
-    for (int r0 = 0; r0 < NS; r0 += c->nbatch) {
-      const int nb = std::min(c->nbatch, NS - r0);
-      if ((rc = fused_batch(c, r0, nb))) return rc;
-    }
+    if ((rc = fused_serial(c))) return rc;
     PRC_CHECK(c, launch_sigma_finish(c->d_part, c->nblk_fz, NS, invN3, c->d_sigma_grid, s), "sigma_grid");
   }
   return PRC_D2Z_OK;
@@ -487,7 +503,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   size_t freeb = 0, totalb = 0;
   const bool have_info = hipMemGetInfo(&freeb, &totalb) == hipSuccess;
   auto tbytes_for = [&](int nb) { return std::max((size_t)2 * sizeof(float2) * c->nspec, tstore * c->nspec * nb); };
-  // default batch: the largest radii-per-launch (<= 8) whose intermediates fit
+  // default batch: the largest radii-per-launch (<= 16) whose intermediates fit
   if (p->mode != PRC_D2Z_PLAIN_FP32 && !p->radii_per_launch && have_info)
     while (c->nbatch > 1 && fixed + tbytes_for(c->nbatch) > freeb) --c->nbatch;
   c->tbytes = tbytes_for(c->nbatch);
@@ -804,7 +820,7 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
     c->timing = was_timing;
     return rc;
   }
-  const int nr = std::min(c->p.Nscales, 8);
+  const int nr = std::min(c->p.Nscales, c->nbatch_cap);
   std::vector<int> nbs;
   for (int nb = c->nbatch_cap; nb >= 1; nb = (nb > 1) ? nb / 2 : 0) nbs.push_back(nb);
   // the resident-capacity grid chosen at create, its double, and fixed sizes
@@ -825,7 +841,10 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
       c->nblk_fz = g;
       for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up
         (void)hipEventRecord(a, c->stream);
-        for (int r0 = 0; r0 < nr && !rc; r0 += nb) rc = fused_batch(c, r0, std::min(nb, nr - r0));
+        for (int r0 = 0; r0 < nr && !rc; r0 += nb) {
+          const int n = std::min(nb, nr - r0);
+          if (!(rc = fused_x(c, r0, n, c->d_T, c->stream))) rc = fused_yz(c, r0, n, c->d_T, c->stream);
+        }
         (void)hipEventRecord(b, c->stream);
       }
       if (rc) break;
