@@ -150,6 +150,50 @@ __global__ void __launch_bounds__(P::T* C)
   for (int e = 0; e < E; ++e, op += ostep) st(op, 0, cscale(v[e], oscale));
 }
 
+// Persistent in-place y pass over FP16 intermediates (the mixed path): block b
+// walks tiles vb = b + k gridDim.x (through the same XCD-aware map) and loads
+// tile vb + gridDim.x into registers, as FP16 pairs, while tile vb is
+// transformed and stored, so one tile's loads overlap the previous tile's
+// arithmetic.  pencil_fft ends with a barrier after its last LDS read, so the
+// LDS is free for the next tile.
+template <class P, int C>
+__global__ void __launch_bounds__(P::T* C)
+    k_tile_yh(__half2* data, long long sA, long long sO, long long sCB, int nOuter, long long ntiles, float oscale,
+              const float2* __restrict__ tw) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  __shared__ float2 smem[L * C];
+  const int tid = threadIdx.x;
+  const int c = tid % C, i = tid / C;
+  const long long istep = (long long)T * sA;
+  const long long lane_off = c + (long long)i * sA;
+  auto tile_base = [&](long long vb) {
+    const long long tile = xcd_tile(vb, ntiles);
+    return (tile / nOuter) * sCB + (tile % nOuter) * sO + lane_off;
+  };
+  uint32_t raw[E];
+  auto fetch = [&](long long vb) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(data) + tile_base(vb);
+#pragma unroll
+    for (int e = 0; e < E; ++e) raw[e] = p[e * istep];
+  };
+  long long vb = blockIdx.x;
+  if (vb < ntiles) fetch(vb);
+  for (; vb < ntiles; vb += gridDim.x) {
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      __half2 h;
+      *reinterpret_cast<uint32_t*>(&h) = raw[e];
+      v[e] = __half22float2(h);
+    }
+    if (vb + gridDim.x < ntiles) fetch(vb + gridDim.x);
+    pencil_fft<P, +1>(v, i, tw, LdsPencil{smem + c, C});
+    __half2* op = data + tile_base(vb);
+#pragma unroll
+    for (int e = 0; e < E; ++e) st(op, e * istep, cscale(v[e], oscale));
+  }
+}
+
 // ---------------------------------------------------- inverse z pass (c2r)
 // Rows (x, y) gathered from the T layout -> dense real rows.
 template <class P, int C, class TS>
@@ -603,6 +647,28 @@ static hipError_t tile_impl(const TI* in, TO* out, const TileMap& mp, float osca
   return hipGetLastError();
 }
 
+// Blocks of kernel f (threads per block) resident on the current device at once
+template <class F>
+static int resident_blocks(F f, int threads) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, 0);
+  return (e == hipSuccess && cus > 0 && per_cu > 0) ? cus * per_cu : 1024;
+}
+
+template <int N>
+static hipError_t tile_yh_impl(__half2* d, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
+  using P = Plan<N>;
+  constexpr int C = Blk<N>::C;
+  const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
+  static const int resident = resident_blocks(k_tile_yh<P, C>, P::T * C);
+  const long long g = ntiles < resident ? ntiles : resident;
+  hipLaunchKernelGGL((k_tile_yh<P, C>), dim3((unsigned)g), dim3(P::T * C), 0, s, d, mp.in_sA, mp.in_sO, mp.sCB,
+                     mp.nOuter, ntiles, oscale, tw);
+  return hipGetLastError();
+}
+
 template <int N>
 static hipError_t unpack_impl(const float2* S, float2* D0, float2* DN, hipStream_t s) {
   const long long n = (long long)N * N;
@@ -766,10 +832,12 @@ hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const Tile
 
 hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
   __half2* d = static_cast<__half2*>(data);
+  // in place along y only (the persistent kernel reads and writes one map)
+  if (mp.in_sA != mp.out_sA || mp.in_sO != mp.out_sO) return hipErrorInvalidValue;
   switch (N) {
 #define X(NN) \
   case NN:    \
-    return tile_impl<NN, +1, __half2, __half2>(d, d, mp, oscale, tw, s);
+    return tile_yh_impl<NN>(d, mp, oscale, tw, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
