@@ -30,6 +30,7 @@ namespace prc {
 namespace {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kN = 1024;         // transform length and grid size
@@ -111,6 +112,21 @@ __device__ __forceinline__ void store_d(uint32_t* buf, const float4_t& d0, const
 constexpr int kMatSets = 2 + 2 * 16;
 __device__ __forceinline__ int mat_set_a(int h) { return h; }
 __device__ __forceinline__ int mat_set_b(int k, int h) { return 2 + 2 * k + h; }
+
+// Real-output glue of the half-length c2r in packed FP16: Z[k] = P a + Q conj(m)
+// with a = X[k], m = X[(L - k) mod L], P = 1 + i w^k, Q = 1 - i w^k (w = exp(2 pi
+// i / N)), the same as (a + conj m) + i w^k (a - conj m); P = 0, Q = 1 + i at
+// k = 0 give the packed slot's (a.x + a.y, a.x - a.y).  Four packed FMAs, the
+// swaps and signs in their operand modifiers.
+__device__ __forceinline__ uint32_t c2r_glue_h2(uint32_t av, uint32_t mv, uint32_t pv, uint32_t qv) {
+  const half2_t a = __builtin_bit_cast(half2_t, av), m = __builtin_bit_cast(half2_t, mv);
+  const half2_t P = __builtin_bit_cast(half2_t, pv), Q = __builtin_bit_cast(half2_t, qv);
+  half2_t z = a * P.xx;                                 // (P.x a.x, P.x a.y)
+  z = __builtin_elementwise_fma(a.yx, (half2_t){-P.y, P.y}, z);  // (-P.y a.y, P.y a.x)
+  z = __builtin_elementwise_fma(m, (half2_t){Q.x, -Q.x}, z);     // (Q.x m.x, -Q.x m.y)
+  z = __builtin_elementwise_fma(m.yx, Q.yy, z);                  // (Q.y m.y, Q.y m.x)
+  return __builtin_bit_cast(uint32_t, z);
+}
 
 __global__ void __launch_bounds__(64) k_mfma_mats(half8_t* __restrict__ out) {
   const int set = blockIdx.x, l = threadIdx.x;
@@ -279,7 +295,7 @@ __global__ void __launch_bounds__(512)
   __shared__ uint32_t bufX[kRowsZ * kXStride];  // also holds stage B's output (8 x 516)
   __shared__ uint32_t bufY[kRowsZ * kY1Stride];
   static_assert(kY2Stride <= kXStride, "stage B output must fit in the raw buffer");
-  __shared__ float2 s_twf[kLz];      // exp(2 pi i k / N), k < N/2 (c2r glue)
+  __shared__ uint2 s_pq[kLz];        // glue constants P, Q (FP16 pairs), k < N/2
   __shared__ float2 s_tw2[kLz / 2];  // exp(2 pi i b / (N/2)) (stage C)
   __shared__ double red[kRowsZ][kFusedMaxRadii];
   // operand matrices in the A layout, per lane: stage A [h][lane], stage B
@@ -292,7 +308,11 @@ __global__ void __launch_bounds__(512)
   const int g = l >> 4, n = l & 15;
   const int rowA = n & 7, sel = n >> 3;  // MFMA column n = (row, butterfly sel)
   if (tid < kRowsZ * kFusedMaxRadii) red[tid / kFusedMaxRadii][tid % kFusedMaxRadii] = 0.0;
-  for (int q = tid; q < kLz; q += 512) s_twf[q] = tw_full[q];
+  for (int q = tid; q < kLz; q += 512) {
+    const float2 w = tw_full[q];
+    s_pq[q] = q == 0 ? make_uint2(pack_h2(0.f, 0.f), pack_h2(1.f, 1.f))
+                     : make_uint2(pack_h2(1.f - w.y, w.x), pack_h2(1.f + w.y, -w.x));
+  }
   for (int q = tid; q < kLz / 2; q += 512) s_tw2[q] = tw_full[2 * q];
   if (w < 2) s_gA[w][l] = mats[mat_set_a(w) * 64 + l];
 #pragma unroll
@@ -343,22 +363,17 @@ __global__ void __launch_bounds__(512)
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int b = 4 * w + 2 * t + sel;
-        float2 zz[4];
+        union {
+          uint32_t u[4];
+          half8_t h;
+        } zz;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = b + 32 * (4 * g + j);
-          const float2 a = unpack_h2(bufX[zx_at(rowA, k)]);
-          if (k == 0) {
-            zz[j] = make_float2(a.x + a.y, a.x - a.y);
-          } else {
-            const float2 mk = unpack_h2(bufX[zx_at(rowA, kLz - k)]);
-            const float2 bc = make_float2(mk.x, -mk.y);
-            const float2 sm = cadd(a, bc), df = csub(a, bc);
-            const float2 wd = cmul(s_twf[k], df);
-            zz[j] = make_float2(sm.x - wd.y, sm.y + wd.x);
-          }
+          const uint2 pq = s_pq[k];
+          zz.u[j] = c2r_glue_h2(bufX[zx_at(rowA, k)], bufX[zx_at(rowA, (kLz - k) & (kLz - 1))], pq.x, pq.y);
         }
-        const half8_t bop = pack_b(zz);
+        const half8_t bop = zz.h;
         const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gA[0][l + zo], bop, z4, 0, 0, 0);
         const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(s_gA[1][l + zo], bop, z4, 0, 0, 0);
         const int p0 = 16 * b + g;
