@@ -145,6 +145,15 @@ __device__ __forceinline__ float barrier_z_scaled_f32(float d, float sc, float A
   return fmaf(A, fmaf(d, sc, B), -1.0f);
 }
 
+// Running maximum of barrier values for zreion, which starts at 0 and is only
+// raised: values <= 0 never change it, so the maximum may start at 0, and for
+// non-negative floats the signed-integer order of the bit patterns is the float
+// order (a negative z has the sign bit set and compares below all of them).  One
+// integer max, where fmaxf of a loop-carried value costs a canonicalizing max too.
+__device__ __forceinline__ float zmax_nonneg(float best, float z) {
+  return __int_as_float(max(__float_as_int(best), __float_as_int(z)));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

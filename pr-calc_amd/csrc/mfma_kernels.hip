@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(512)
     float bR0[8], bR1[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      best[e] = make_float2(-INFINITY, -INFINITY);
+      best[e] = make_float2(0.0f, 0.0f);  // zreion >= 0: see zmax_nonneg
       bR0[e] = 0.f;
       bR1[e] = 0.f;
     }
@@ -436,8 +436,8 @@ __global__ void __launch_bounds__(512)
             bR1[e] = R;
           }
         } else {
-          best[e].x = fmaxf(best[e].x, z0);
-          best[e].y = fmaxf(best[e].y, z1);
+          best[e].x = zmax_nonneg(best[e].x, z0);
+          best[e].y = zmax_nonneg(best[e].y, z1);
         }
       }
       const double wsum = wave_sum((double)acc * sc2);

@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(256)
     float bR0[E], bR1[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      best[e] = make_float2(-INFINITY, -INFINITY);
+      best[e] = make_float2(0.0f, 0.0f);  // zreion >= 0: see zmax_nonneg
       bR0[e] = 0.f;
       bR1[e] = 0.f;
     }
@@ -478,9 +478,9 @@ __global__ void __launch_bounds__(256)
             best[e].y = z1;
             bR1[e] = R;
           }
-        } else {  // same result as the strict comparison (z is never NaN or -0)
-          best[e].x = fmaxf(best[e].x, z0);
-          best[e].y = fmaxf(best[e].y, z1);
+        } else {  // same zreion as the strict comparison (z is never NaN)
+          best[e].x = zmax_nonneg(best[e].x, z0);
+          best[e].y = zmax_nonneg(best[e].y, z1);
         }
       }
       const double wsum = wave_sum(valid ? (double)acc * sc2 : 0.0);
