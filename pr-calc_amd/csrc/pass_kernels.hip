@@ -150,15 +150,15 @@ __global__ void __launch_bounds__(P::T* C)
   for (int e = 0; e < E; ++e, op += ostep) st(op, 0, cscale(v[e], oscale));
 }
 
-// Persistent in-place y pass over FP16 intermediates (the mixed path): block b
-// walks tiles vb = b + k gridDim.x (through the same XCD-aware map) and loads
-// tile vb + gridDim.x into registers, as FP16 pairs, while tile vb is
-// transformed and stored, so one tile's loads overlap the previous tile's
+// Persistent in-place tile pass (the y passes; TS = float2 or __half2 storage):
+// block b walks tiles vb = b + k gridDim.x (through the same XCD-aware map) and
+// loads tile vb + gridDim.x into registers, in the storage type, while tile vb
+// is transformed and stored, so one tile's loads overlap the previous tile's
 // arithmetic.  pencil_fft ends with a barrier after its last LDS read, so the
 // LDS is free for the next tile.
-template <class P, int C>
+template <class P, int C, int DIR, class TS>
 __global__ void __launch_bounds__(P::T* C)
-    k_tile_yh(__half2* data, long long sA, long long sO, long long sCB, int nOuter, long long ntiles, float oscale,
+    k_tile_ip(TS* data, long long sA, long long sO, long long sCB, int nOuter, long long ntiles, float oscale,
               const float2* __restrict__ tw) {
   constexpr int L = P::L, T = P::T, E = P::E;
   __shared__ float2 smem[L * C];
@@ -170,9 +170,9 @@ __global__ void __launch_bounds__(P::T* C)
     const long long tile = xcd_tile(vb, ntiles);
     return (tile / nOuter) * sCB + (tile % nOuter) * sO + lane_off;
   };
-  uint32_t raw[E];
+  TS raw[E];
   auto fetch = [&](long long vb) {
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(data) + tile_base(vb);
+    const TS* p = data + tile_base(vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) raw[e] = p[e * istep];
   };
@@ -181,14 +181,10 @@ __global__ void __launch_bounds__(P::T* C)
   for (; vb < ntiles; vb += gridDim.x) {
     float2 v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      __half2 h;
-      *reinterpret_cast<uint32_t*>(&h) = raw[e];
-      v[e] = __half22float2(h);
-    }
+    for (int e = 0; e < E; ++e) v[e] = ld(raw + e, 0);
     if (vb + gridDim.x < ntiles) fetch(vb + gridDim.x);
-    pencil_fft<P, +1>(v, i, tw, LdsPencil{smem + c, C});
-    __half2* op = data + tile_base(vb);
+    pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
+    TS* op = data + tile_base(vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) st(op, e * istep, cscale(v[e], oscale));
   }
@@ -657,15 +653,15 @@ static int resident_blocks(F f, int threads) {
   return (e == hipSuccess && cus > 0 && per_cu > 0) ? cus * per_cu : 1024;
 }
 
-template <int N>
-static hipError_t tile_yh_impl(__half2* d, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
+template <int N, int DIR, class TS>
+static hipError_t tile_ip_impl(TS* d, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
   using P = Plan<N>;
   constexpr int C = Blk<N>::C;
   const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
-  static const int resident = resident_blocks(k_tile_yh<P, C>, P::T * C);
+  static const int resident = resident_blocks(k_tile_ip<P, C, DIR, TS>, P::T * C);
   const long long g = ntiles < resident ? ntiles : resident;
-  hipLaunchKernelGGL((k_tile_yh<P, C>), dim3((unsigned)g), dim3(P::T * C), 0, s, d, mp.in_sA, mp.in_sO, mp.sCB,
-                     mp.nOuter, ntiles, oscale, tw);
+  hipLaunchKernelGGL((k_tile_ip<P, C, DIR, TS>), dim3((unsigned)g), dim3(P::T * C), 0, s, d, mp.in_sA, mp.in_sO,
+                     mp.sCB, mp.nOuter, ntiles, oscale, tw);
   return hipGetLastError();
 }
 
@@ -818,6 +814,9 @@ hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const
 
 hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const TileMap& mp, const float2* tw,
                        hipStream_t s) {
+  // FP32 tiles keep one tile per block: the persistent kernel of the FP16
+  // path measured slower for FP32 pencils of 8 values (N = 512) and spills for
+  // 16 (N = 1024, 2048)
   switch (N) {
 #define X(NN)                                                                          \
   case NN:                                                                             \
@@ -837,7 +836,7 @@ hipError_t launch_tile_h(int N, void* data, const TileMap& mp, float oscale, con
   switch (N) {
 #define X(NN) \
   case NN:    \
-    return tile_yh_impl<NN>(d, mp, oscale, tw, s);
+    return tile_ip_impl<NN, +1, __half2>(d, mp, oscale, tw, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
