@@ -138,12 +138,17 @@ __device__ __forceinline__ float barrier_z_scaled(float d, double sc, double A, 
   return (float)(p - 1.0);
 }
 
-// FP32 form for the mixed-precision path: sc is a power of two, so each fmaf
-// rounds once; |error| <= (|z| + 1) 2^-23, four orders below that path's own
-// FP16 error
-__device__ __forceinline__ float barrier_z_scaled_f32(float d, float sc, float A, float B) {
-  return fmaf(A, fmaf(d, sc, B), -1.0f);
+// FP32 form for the mixed-precision path, one FMA per cell: z = A' d + B' with
+// A' = A sc (exact: A is a float and sc a power of two) and B' = A B - 1
+// rounded once from double; |error| <= (|z| + |B'|) 2^-24 (~2e-6 for these
+// barriers), far below that path's own FP16 error
+struct BarrierF32 {
+  float a, b;
+};
+__device__ __forceinline__ BarrierF32 barrier_f32(double A, double B, float sc) {
+  return BarrierF32{(float)(A * (double)sc), (float)(A * B - 1.0)};
 }
+__device__ __forceinline__ float barrier_z_f32(float d, BarrierF32 k) { return fmaf(k.a, d, k.b); }
 
 // Running maximum of barrier values for zreion, which starts at 0 and is only
 // raised: values <= 0 never change it, so the maximum may start at 0, and for

@@ -418,13 +418,13 @@ __global__ void __launch_bounds__(512)
         v[m + 4] = csub(u0, u1);
       }
       __syncthreads();  // the raw buffer is reloaded next radius
-      const float A = (float)rc[r0 + q].A, B = (float)rc[r0 + q].B;
+      const BarrierF32 bk = barrier_f32(rc[r0 + q].A, rc[r0 + q].B, iscale);
       const float R = rc[r0 + q].R;
       float acc = 0.0f;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const float z0 = barrier_z_scaled_f32(v[e].x, iscale, A, B);
-        const float z1 = barrier_z_scaled_f32(v[e].y, iscale, A, B);
+        const float z0 = barrier_z_f32(v[e].x, bk);
+        const float z1 = barrier_z_f32(v[e].y, bk);
         acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
         if constexpr (RB) {
           if (z0 > best[e].x) {

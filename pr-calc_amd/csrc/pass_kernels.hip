@@ -456,14 +456,14 @@ __global__ void __launch_bounds__(256)
       // barrier's FMA and to the sum of squares, where it is exact
       // (the FP16-storage path evaluates it in FP32, far inside its own error)
       const double A = rc[r0 + q].A, B = rc[r0 + q].B;
-      const float Af = (float)A, Bf = (float)B;
+      const BarrierF32 bk = barrier_f32(A, B, iscale);
       const float R = rc[r0 + q].R;
       float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         constexpr bool kF32Barrier = std::is_same<TS, __half2>::value;
-        const float z0 = kF32Barrier ? barrier_z_scaled_f32(v[e].x, iscale, Af, Bf) : barrier_z_scaled(v[e].x, sc, A, B);
-        const float z1 = kF32Barrier ? barrier_z_scaled_f32(v[e].y, iscale, Af, Bf) : barrier_z_scaled(v[e].y, sc, A, B);
+        const float z0 = kF32Barrier ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
+        const float z1 = kF32Barrier ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
         acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
         if constexpr (RB) {
           if (z0 > best[e].x) {
