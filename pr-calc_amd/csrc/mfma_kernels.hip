@@ -1,9 +1,10 @@
-// Matrix-core (MFMA) variant of the fused x pass for the mixed path at
-// N = 1024 (FP16 intermediates): the two radix-16 stages of the 1024-point
-// inverse transform (1024 = 16 * 16 * 4) run as 32 x 32 real matrix products
-// on v_mfma_f32_16x16x32_f16 (FP16 operands, FP32 accumulation), the radix-4
-// stage on the vector ALUs.  Reference: delta2zreion_d2z.C:58-92 (filter and
-// c2r), with the same W(kR) table and the same output layout as k_fx.
+// Matrix-core (MFMA) variants of the fused x and z passes for the mixed path
+// at N = 1024 (FP16 intermediates).  x pass: the two radix-16 stages of the
+// 1024-point inverse transform (1024 = 16 * 16 * 4) run as 32 x 32 real matrix
+// products on v_mfma_f32_16x16x32_f16 (FP16 operands, FP32 accumulation), the
+// radix-4 stage on the vector ALUs.  Reference: delta2zreion_d2z.C:58-92
+// (filter and c2r), with the same W(kR) table and the same output layout as
+// k_fx.  The z pass (below) does the same for its 512-point half-length c2r.
 //
 // Stockham formulation (as fft_core.h): stage (p, R) takes butterfly b's
 // inputs x[b + r L/R], multiplies input r by w^(r (b mod p)) with w =
@@ -264,11 +265,12 @@ __global__ void __launch_bounds__(kThreads)
 // fused, as k_fz (reference delta2zreion_d2z.C:92-146,206-216), for blocks of
 // 8 consecutive z rows (512 B contiguous per kz block in the T layout).  Per
 // radius: the raw FP16 rows are staged in LDS; stage A forms the complex
-// sequence Z[k] of the half-length c2r from X[k] and X[L-k] (FP32) and runs
-// radix 16 on the matrix cores; stage B (radix 16, p = 16) groups the 16
+// sequence Z[k] of the half-length c2r from X[k] and X[L-k] (packed FP16) and
+// runs radix 16 on the matrix cores; stage B (radix 16, p = 16) groups the 16
 // butterflies {k, k+16} x 8 rows that share k = b mod 16; stage C (radix 2,
-// p = 256) leaves row w in wave w, which evaluates d2z() in FP64, keeps the
-// maximum over the batch in registers and the per-radius sums of squares.
+// p = 256, FP32) leaves row w in wave w, which evaluates d2z() in FP32 (one FMA
+// per cell), keeps the maximum over the batch in registers and the per-radius
+// sums of squares.
 constexpr int kLz = kN / 2;   // complex points per z row
 constexpr int kRowsZ = 8;     // rows per block
 // Padded layouts (bank model: conflict-free except stage B's writes, 2-way);
