@@ -150,6 +150,15 @@ int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */)
  * through the context's transform path. */
 int prc_d2z_smoothed_field(prc_d2z_ctx* ctx, int r, float* out);
 
+/* Linear displacement field of the context's density field, component axis
+ * (0 = x, 1 = y, 2 = z): psi_a = IFFT(i k_a / k^2 FFT(delta)) / N^3 with
+ * k = 2 pi n / BoxSize (BoxSize as in the parameters), k = 0 and the
+ * component's own Nyquist plane zeroed -- Delta2Velocity's xdv_a
+ * (c++/src/delta2velocity_map.C:8-97, its two N^-1.5 normalisations
+ * included), which the map programs scale into velocities.  FP32 transforms;
+ * out: N^3 floats in `layout` (enum prc_d2z_layout). */
+int prc_d2z_displacement(prc_d2z_ctx* ctx, int axis, float* out, int layout);
+
 /* Launch shape of the fused modes: radii per fused batch (1..max_radii, bounded
  * by the intermediate buffers allocated at create) and the grid of the fused
  * z pass (1..4096 blocks). */

@@ -21,7 +21,10 @@ repository root):
   (:92-93), sigma_grid (c++/src/arrayoperations.C:4-35), the barrier and the
   strict running maximum (delta2zreion_d2z.C:106-146);
 * ``GetHistory`` and ``tau`` (c++/src/bubblehistory_d2z.C:5-41,
-  c++/src/cosmology.C:52-83) with the reference's float accumulation.
+  c++/src/cosmology.C:52-83) with the reference's float accumulation;
+* the linear displacement field of ``Delta2Velocity``
+  (c++/src/delta2velocity_map.C:8-97), the downstream use of the same
+  transforms.
 
 Third-party arithmetic the reference calls, restated here:
 * GSL ``gsl_integration_qags`` -> ``scipy.integrate.quad`` (both are QUADPACK
@@ -373,6 +376,37 @@ def smooth(dhat, kmag, R, workers=None):
         import scipy.fft
         return scipy.fft.irfftn(dhat * W, s=(N, N, N), axes=(0, 1, 2), workers=workers)
     return np.fft.irfftn(dhat * W, s=(N, N, N), axes=(0, 1, 2))
+
+
+def displacement(delta, box, axis, workers=None):
+    """Linear displacement component psi_a = c2r(i k_a / k^2 * r2c(delta)) / N^3
+    with k = n 2 pi / box (signed n: n >= N/2 -> n - N), k = 0 and the
+    component's own Nyquist plane zeroed: Delta2Velocity's xdv_a
+    (c++/src/delta2velocity_map.C:8-97; its two NormalizeArray(.., 1.5) calls,
+    :24,90-92, give 1/N^3 in all)."""
+    N = delta.shape[0]
+    dk = 2.0 * math.pi / f64(box)
+    n = np.fft.fftfreq(N, 1.0 / N)                  # 0..N/2-1, -N/2..-1
+    nz = np.arange(N // 2 + 1, dtype=np.float64)     # 0..N/2
+    kx = (dk * n)[:, None, None]
+    ky = (dk * n)[None, :, None]
+    kz = (dk * nz)[None, None, :]
+    k2 = kx * kx + ky * ky + kz * kz
+    ka = np.broadcast_to((kx, ky, kz)[axis], k2.shape)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m = np.where(k2 > 0, 1j * ka / np.where(k2 > 0, k2, 1.0), 0.0)
+    nyq = (np.abs(n) == N // 2)
+    if axis == 0:
+        m[nyq, :, :] = 0.0
+    elif axis == 1:
+        m[:, nyq, :] = 0.0
+    else:
+        m[:, :, N // 2] = 0.0
+    if workers:
+        import scipy.fft
+        dhat = scipy.fft.rfftn(f64(delta), axes=(0, 1, 2), workers=workers)
+        return scipy.fft.irfftn(dhat * m, s=(N, N, N), axes=(0, 1, 2), workers=workers)
+    return np.fft.irfftn(np.fft.rfftn(f64(delta)) * m, s=(N, N, N), axes=(0, 1, 2))
 
 
 def sigma_grid(dR):

@@ -94,6 +94,10 @@ hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const Tile
 hipError_t launch_unpack(int N, const float2* S, float2* D0, float2* DN, hipStream_t s);
 hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
                          float invN3, hipStream_t s);
+// i k_a / |k|^2 * scale * S, component a = axis (0, 1, 2), k = n dk (the
+// displacement field's spectrum, S layout as the filter's output)
+hipError_t launch_disp(int N, const float2* S, const float2* D0, const float2* DN, float2* out, int axis, float dk,
+                       float scale, hipStream_t s);
 hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const float2* twh, const float2* twf,
                           hipStream_t s);
 // in-place tiled pass over FP16 intermediates (mixed path), output scaled

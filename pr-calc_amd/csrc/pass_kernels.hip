@@ -259,6 +259,44 @@ __global__ void k_filter(const float2* __restrict__ S, const float2* __restrict_
   }
 }
 
+// ---------------------------------------- displacement (linear velocity) field
+// T = i k_a / |k|^2 * scale * S for component a (0 = x, 1 = y, 2 = z), as
+// Delta2Velocity forms xdv_a (delta2velocity_map.C:27-79): k = n dk with the
+// signed index n (n >= N/2 -> n - N), the component's own Nyquist plane and
+// k = 0 zeroed; the kz = 0 slot is rebuilt as m0 D0 + i mN DN (m = i c).
+template <int C>
+__global__ void k_disp(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+                       float2* __restrict__ out, int N, int axis, float dk, float scale) {
+  const int Nh = N / 2;
+  const long long total = (long long)N * N * Nh;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const SIndex k = s_index<C>(id, N);
+    const int nx = k.kx >= Nh ? k.kx - N : k.kx, ny = k.ky >= Nh ? k.ky - N : k.ky;
+    // k_a / |k|^2 * scale at (nx, ny, nz), in the reference's float operations
+    auto coef = [&](int nz) -> float {
+      const int na = axis == 0 ? nx : (axis == 1 ? ny : nz);
+      if (na == Nh || na == -Nh) return 0.0f;
+      const float kx = nx * dk, ky = ny * dk, kz = nz * dk;
+      const float rk2 = kx * kx + ky * ky + kz * kz;
+      if (rk2 == 0.0f) return 0.0f;
+      return (na * dk) / rk2 * scale;
+    };
+    float2 r;
+    if (k.kz != 0) {
+      const float c = coef(k.kz);
+      const float2 a = S[id];
+      r = make_float2(-c * a.y, c * a.x);
+    } else {
+      const long long pl = (long long)k.kx * N + k.ky;
+      const float c0 = coef(0), cn = coef(Nh);
+      const float2 a = D0[pl], b = DN[pl];
+      r = make_float2(-c0 * a.y - cn * b.x, c0 * a.x - cn * b.y);
+    }
+    out[id] = r;
+  }
+}
+
 // ============================================================ fused path
 // First inverse pass (x) with the filter fused.  A block owns CF of the C
 // columns of one spectrum tile (ky, kz block); sibling sub-tiles are adjacent
@@ -673,6 +711,16 @@ static hipError_t unpack_impl(const float2* S, float2* D0, float2* DN, hipStream
 }
 
 template <int N>
+static hipError_t disp_impl(const float2* S, const float2* D0, const float2* DN, float2* out, int axis, float dk,
+                            float scale, hipStream_t s) {
+  const long long n = (long long)N * N * (N / 2);
+  long long g = (n + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  hipLaunchKernelGGL((k_disp<Blk<N>::C>), dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, out, N, axis, dk, scale);
+  return hipGetLastError();
+}
+
+template <int N>
 static hipError_t filter_impl(const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
                               float invN3, hipStream_t s) {
   const long long n = (long long)N * N * (N / 2);
@@ -862,6 +910,20 @@ hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2*
 #define X(NN) \
   case NN:    \
     return filter_impl<NN>(S, D0, DN, out, dkR, invN3, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_disp(int N, const float2* S, const float2* D0, const float2* DN, float2* out, int axis, float dk,
+                       float scale, hipStream_t s) {
+  if (axis < 0 || axis > 2) return hipErrorInvalidValue;
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return disp_impl<NN>(S, D0, DN, out, axis, dk, scale, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

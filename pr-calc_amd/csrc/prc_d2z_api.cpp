@@ -774,6 +774,36 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
   return copy_field_out(c, real, out, PRC_D2Z_DENSE);
 }
 
+// Delta2Velocity (delta2velocity_map.C:8-97): psi_a = IFFT(i k_a / k^2 FFT(delta))
+// with the reference's N^-1.5 after each transform, i.e. 1/N^3 in all, and
+// k = n 2 pi / BoxSize (BoxSize as given).  The context's spectrum is that of
+// delta / DInit (the d2z input scaling), so DInit is folded into the scale.
+int prc_d2z_displacement(prc_d2z_ctx* c, int axis, float* out, int layout) {
+  if (!c || !out) return PRC_D2Z_ERR_BAD_ARG;
+  if (axis < 0 || axis > 2) return c->fail(PRC_D2Z_ERR_BAD_ARG, "axis must be 0 (x), 1 (y) or 2 (z)");
+  if (layout != PRC_D2Z_DENSE && layout != PRC_D2Z_PADDED) return c->fail(PRC_D2Z_ERR_BAD_ARG, "unknown layout");
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  (void)hipSetDevice(c->p.device);
+  const bool was_timing = c->timing;
+  c->timing = false;
+  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c, c->sc.DInit);
+  c->timing = was_timing;
+  if (rc) return rc;
+  const int N = c->N;
+  hipStream_t s = c->stream;
+  float2* T1 = c->d_T;
+  float2* T2 = c->d_T + c->nspec;  // the intermediates hold at least two FP32 spectra
+  const TileMap xm = tile_map(N, Axis::kXStoT), ym = tile_map(N, Axis::kYInPlace);
+  const float dk = (float)(2.0 * M_PI / (double)c->p.BoxSize);
+  const float scale = (float)((double)c->sc.DInit / ((double)N * N * N));
+  PRC_CHECK(c, launch_disp(N, c->d_S, c->d_D0, c->d_DN, T1, axis, dk, scale, s), "displacement multiply");
+  PRC_CHECK(c, launch_tile(N, +1, T1, T2, xm, c->d_st_full, s), "inverse x pass");
+  PRC_CHECK(c, launch_tile(N, +1, T2, T2, ym, c->d_st_full, s), "inverse y pass");
+  float* real = reinterpret_cast<float*>(T1);
+  PRC_CHECK(c, launch_inv_z(N, T2, real, c->d_st_half, c->d_tw_full, s), "inverse z pass");
+  return copy_field_out(c, real, out, layout);
+}
+
 int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {
   if (!c) return PRC_D2Z_ERR_BAD_ARG;
   c->timing = enable != 0;

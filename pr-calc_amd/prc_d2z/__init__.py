@@ -69,7 +69,7 @@ EXPORTS = [
     "prc_d2z_last_error", "prc_d2z_set_delta", "prc_d2z_set_delta_device", "prc_d2z_run",
     "prc_d2z_run_async", "prc_d2z_sync", "prc_d2z_get_zreion", "prc_d2z_get_rbubble",
     "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
-    "prc_d2z_smoothed_field", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
+    "prc_d2z_smoothed_field", "prc_d2z_displacement", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
 ]
@@ -107,6 +107,7 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_zreion_device": (C.c_int, [vp, C.POINTER(vp)]),
         "prc_d2z_history": (C.c_int, [vp, up]),
         "prc_d2z_smoothed_field": (C.c_int, [vp, C.c_int, fp]),
+        "prc_d2z_displacement": (C.c_int, [vp, C.c_int, fp, C.c_int]),
         "prc_d2z_set_timing": (C.c_int, [vp, C.c_int]),
         "prc_d2z_get_timing": (C.c_int, [vp, C.POINTER(CTiming)]),
         "prc_d2z_device_bytes": (C.c_size_t, [vp]),
@@ -244,6 +245,14 @@ class D2Z:
         N = self.N
         out = np.empty((N, N, N), dtype=np.float32)
         self._check(self.lib.prc_d2z_smoothed_field(self._ctx, r, out))
+        return out
+
+    def displacement(self, axis: int, layout: int = DENSE) -> np.ndarray:
+        """Linear displacement field component axis (0, 1, 2) of the current
+        density field, as Delta2Velocity (delta2velocity_map.C:8-97)."""
+        N = self.N
+        out = np.empty((N, N, N + 2) if layout == PADDED else (N, N, N), dtype=np.float32)
+        self._check(self.lib.prc_d2z_displacement(self._ctx, axis, out, layout))
         return out
 
     def set_launch_config(self, radii_per_launch: int, z_blocks: int):

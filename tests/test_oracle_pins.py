@@ -330,3 +330,37 @@ def test_radius_schedule_endpoints_and_spacing():
     assert R[0] == pytest.approx(200.0 / 0.68 / 128, rel=1e-6)
     assert R[-1] == pytest.approx(50.0 / 0.68, rel=1e-5)
     np.testing.assert_allclose(np.diff(np.log(R)), np.log(R[-1] / R[0]) / 31, rtol=1e-4)
+
+
+# ---------------------------------------------------- displacement (Delta2Velocity)
+def _plane(N, n):
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    return 2.0 * np.pi * (n[0] * i + n[1] * j + n[2] * k) / N
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_displacement_of_a_plane_wave_is_closed_form(axis):
+    """delta = cos(k.x) gives psi_a = -(k_a / k^2) sin(k.x): pins the sign of i,
+    the 1/N^3 normalisation, the axis order and k = 2 pi n / box."""
+    N, box = 16, 37.0
+    n = (2, -3, 5)
+    ph = _plane(N, n)
+    dk = 2.0 * np.pi / box
+    k = np.array(n, dtype=np.float64) * dk
+    psi = o.displacement(np.cos(ph), box, axis)
+    ref = -(k[axis] / np.dot(k, k)) * np.sin(ph)
+    assert np.max(np.abs(psi - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_displacement_zeroes_the_components_own_nyquist_plane():
+    """x Nyquist mixed with a y mode: the x component is zeroed (the reference
+    skips kx = N/2 for xdv_x), the y component is not."""
+    N, box = 16, 20.0
+    ph = _plane(N, (N // 2, 1, 0))
+    delta = np.cos(ph)
+    dk = 2.0 * np.pi / box
+    kx, ky = dk * N / 2, dk * 1.0
+    assert np.max(np.abs(o.displacement(delta, box, 0))) <= 1e-14
+    ref_y = -(ky / (kx * kx + ky * ky)) * np.sin(ph)
+    assert np.max(np.abs(o.displacement(delta, box, 1) - ref_y)) <= 1e-12 * np.max(np.abs(ref_y))
+    assert np.max(np.abs(o.displacement(delta, box, 2))) <= 1e-14
