@@ -159,6 +159,16 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* ctx, int r, float* out);
  * out: N^3 floats in `layout` (enum prc_d2z_layout). */
 int prc_d2z_displacement(prc_d2z_ctx* ctx, int axis, float* out, int layout);
 
+/* Binned power spectrum of the context's density field, PowerSpectrum
+ * (c++/src/powerspectrum_lmb.C:8-139, transform = 1): modes of the half
+ * spectrum (kz = 0..N/2, each once) of FFT(delta) N^-1.5, k = 2 pi n /
+ * BoxSize in the reference's float arithmetic, bin = (int)((log k - log kmin)
+ * / dlnk) with dlnk = (log kmax - log kmin) / nk.  Per bin (nk <= 512): ps =
+ * (2 pi)^3 <|delta_k|^2> / (dk N)^3, ook2 = <1/k^2>, kmean = <k>, nmodes =
+ * count; empty bins give NaN as the reference's divisions by zero. */
+int prc_d2z_power_spectrum(prc_d2z_ctx* ctx, int nk, float kmin, float kmax, double* ps, double* ook2,
+                           double* kmean, int64_t* nmodes);
+
 /* Launch shape of the fused modes: radii per fused batch (1..max_radii, bounded
  * by the intermediate buffers allocated at create) and the grid of the fused
  * z pass (1..4096 blocks). */

@@ -23,8 +23,9 @@ repository root):
 * ``GetHistory`` and ``tau`` (c++/src/bubblehistory_d2z.C:5-41,
   c++/src/cosmology.C:52-83) with the reference's float accumulation;
 * the linear displacement field of ``Delta2Velocity``
-  (c++/src/delta2velocity_map.C:8-97), the downstream use of the same
-  transforms.
+  (c++/src/delta2velocity_map.C:8-97) and the binned power spectrum of
+  ``PowerSpectrum`` (c++/src/powerspectrum_lmb.C:8-139), downstream uses of
+  the same forward transform.
 
 Third-party arithmetic the reference calls, restated here:
 * GSL ``gsl_integration_qags`` -> ``scipy.integrate.quad`` (both are QUADPACK
@@ -407,6 +408,44 @@ def displacement(delta, box, axis, workers=None):
         dhat = scipy.fft.rfftn(f64(delta), axes=(0, 1, 2), workers=workers)
         return scipy.fft.irfftn(dhat * m, s=(N, N, N), axes=(0, 1, 2), workers=workers)
     return np.fft.irfftn(np.fft.rfftn(f64(delta)) * m, s=(N, N, N), axes=(0, 1, 2))
+
+
+def power_spectrum(delta, box, nk, kmin, kmax):
+    """PowerSpectrum (c++/src/powerspectrum_lmb.C:8-139, transform = 1): modes
+    of the half spectrum (kz = 0..N/2) of FFT(delta) N^-1.5 (:37-38), binned by
+    bin = (int)((log k - log kmin) / dlnk) (:73) with the reference's float
+    set-up (:18-26) and float k (:55-71); per bin the count, (2 pi)^3
+    <|delta_k|^2> / d3k, <1/k^2> and <k> (:78-81,108-110; NaN when empty).
+    The power itself in FP64."""
+    f32 = np.float32
+    N = delta.shape[0]
+    fk = np.fft.rfftn(f64(delta)) * f64(N) ** -1.5
+    dk = f32(2.0 * math.pi / f64(f32(box)))
+    kmaxg = f32(dk * f32(N))
+    d3k = f32(f64(kmaxg) ** 3)
+    lnkmin = np.log(f32(kmin))
+    dlnk = f32((np.log(f32(kmax)) - lnkmin) / f32(nk))
+
+    def axis_k(n):
+        i = np.arange(n)
+        k = (i.astype(f32) * dk).astype(f32)
+        return np.where(i >= N // 2, (k - kmaxg).astype(f32), k)
+
+    kxy, kz = axis_k(N), axis_k(N // 2 + 1)
+    rk2 = ((kxy * kxy)[:, None, None] + (kxy * kxy)[None, :, None]).astype(f32) + (kz * kz)[None, None, :]
+    rk2 = rk2.astype(f32)
+    rk = np.sqrt(rk2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = ((np.log(rk) - lnkmin) / dlnk).astype(f32)
+    sel = (u > -1.0) & (u < nk)
+    b = np.trunc(u[sel]).astype(np.int64)
+    power = (fk.real ** 2 + fk.imag ** 2)[sel]
+    n = np.bincount(b, minlength=nk).astype(np.int64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ps = np.bincount(b, power / f64(d3k), minlength=nk) * (2.0 * math.pi) ** 3 / n
+        ook2 = np.bincount(b, (f32(1.0) / rk2[sel]).astype(np.float64), minlength=nk) / n
+        kmean = np.bincount(b, rk[sel].astype(np.float64), minlength=nk) / n
+    return {"ps": ps, "ook2": ook2, "kmean": kmean, "nmodes": n, "u": u}
 
 
 def sigma_grid(dR):

@@ -98,6 +98,11 @@ hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2*
 // displacement field's spectrum, S layout as the filter's output)
 hipError_t launch_disp(int N, const float2* S, const float2* D0, const float2* DN, float2* out, int axis, float dk,
                        float scale, hipStream_t s);
+// binned power of the half spectrum (count[nk], sums[3][nk]: power * pscale,
+// 1/k^2, k); clears both first (not for graph capture)
+constexpr int kPkMaxBins = 512;
+hipError_t launch_pk(int N, const float2* S, const float2* D0, const float2* DN, float dk, float lnkmin, float dlnk,
+                     int nk, double pscale, unsigned long long* count, double* sums, hipStream_t s);
 hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const float2* twh, const float2* twf,
                           hipStream_t s);
 // in-place tiled pass over FP16 intermediates (mixed path), output scaled

@@ -297,6 +297,66 @@ __global__ void k_disp(const float2* __restrict__ S, const float2* __restrict__ 
   }
 }
 
+// ------------------------------------------------ binned power spectrum
+// PowerSpectrum (powerspectrum_lmb.C:8-139) over the half spectrum kz in
+// [0, N/2] (kz = 0 and N/2 from the unpacked planes D0, DN): per mode the
+// reference's float k (kx = i dk, minus kmax for i >= N/2), rk2 = kx2 + ky2 +
+// kz2, bin = (int)((log(rk) - lnkmin) / dlnk); per bin the count, sum of
+// |field|^2 pscale (pscale = DInit^2 / N^3 / d3k: the field is
+// FFT(delta) N^-1.5 and S holds FFT(delta / DInit)), sum of 1/rk2 and of
+// sqrt(rk2).  Block sums in LDS, then FP64 atomics into out.
+template <int C>
+__global__ void __launch_bounds__(256)
+    k_pk(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN, int N, float dk,
+         float lnkmin, float dlnk, int nk, double pscale, unsigned long long* __restrict__ count,
+         double* __restrict__ sums /* [3][nk]: power, 1/k^2, k */) {
+  __shared__ unsigned s_n[kPkMaxBins];
+  __shared__ double s_p[kPkMaxBins], s_o[kPkMaxBins], s_k[kPkMaxBins];
+  for (int b = threadIdx.x; b < nk; b += blockDim.x) {
+    s_n[b] = 0u;
+    s_p[b] = s_o[b] = s_k[b] = 0.0;
+  }
+  __syncthreads();
+  const int Nh = N / 2;
+  const float kmax = dk * (float)N;
+  const long long plane = (long long)N * N, total = plane * (Nh + 1);
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(id / plane), ky = (int)((id / N) % N), kx = (int)(id % N);
+    float2 v;
+    if (kz == 0)
+      v = D0[(long long)kx * N + ky];
+    else if (kz == Nh)
+      v = DN[(long long)kx * N + ky];
+    else
+      v = S[(((long long)(kz / C) * N + ky) * N + kx) * C + kz % C];
+    float fx = kx * dk, fy = ky * dk, fz = kz * dk;
+    if (kx >= Nh) fx = fx - kmax;
+    if (ky >= Nh) fy = fy - kmax;
+    if (kz >= Nh) fz = fz - kmax;
+    const float rk2 = fx * fx + fy * fy + fz * fz;
+    const float rk = sqrtf(rk2);
+    const float u = (logf(rk) - lnkmin) / dlnk;
+    // the reference's int conversion truncates toward zero: u in (-1, 0) (k
+    // just below kmin) lands in bin 0; k = 0 gives -inf and no bin
+    if (!(u > -1.0f) || u >= (float)nk) continue;
+    const int bin = (int)u;
+    const double power = ((double)v.x * v.x + (double)v.y * v.y) * pscale;
+    atomicAdd(&s_n[bin], 1u);
+    atomicAdd(&s_p[bin], power);
+    atomicAdd(&s_o[bin], (double)(1.0f / rk2));
+    atomicAdd(&s_k[bin], (double)rk);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nk; b += blockDim.x) {
+    if (!s_n[b]) continue;
+    atomicAdd(&count[b], (unsigned long long)s_n[b]);
+    atomicAdd(&sums[b], s_p[b]);
+    atomicAdd(&sums[nk + b], s_o[b]);
+    atomicAdd(&sums[2 * nk + b], s_k[b]);
+  }
+}
+
 // ============================================================ fused path
 // First inverse pass (x) with the filter fused.  A block owns CF of the C
 // columns of one spectrum tile (ky, kz block); sibling sub-tiles are adjacent
@@ -721,6 +781,17 @@ static hipError_t disp_impl(const float2* S, const float2* D0, const float2* DN,
 }
 
 template <int N>
+static hipError_t pk_impl(const float2* S, const float2* D0, const float2* DN, float dk, float lnkmin, float dlnk,
+                          int nk, double pscale, unsigned long long* count, double* sums, hipStream_t s) {
+  const long long n = (long long)N * N * (N / 2 + 1);
+  long long g = (n + 255) / 256;
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL((k_pk<Blk<N>::C>), dim3((unsigned)g), dim3(256), 0, s, S, D0, DN, N, dk, lnkmin, dlnk, nk, pscale,
+                     count, sums);
+  return hipGetLastError();
+}
+
+template <int N>
 static hipError_t filter_impl(const float2* S, const float2* D0, const float2* DN, float2* out, float dkR,
                               float invN3, hipStream_t s) {
   const long long n = (long long)N * N * (N / 2);
@@ -910,6 +981,23 @@ hipError_t launch_filter(int N, const float2* S, const float2* D0, const float2*
 #define X(NN) \
   case NN:    \
     return filter_impl<NN>(S, D0, DN, out, dkR, invN3, s);
+    PRC_FOR_EACH_N(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pk(int N, const float2* S, const float2* D0, const float2* DN, float dk, float lnkmin, float dlnk,
+                     int nk, double pscale, unsigned long long* count, double* sums, hipStream_t s) {
+  if (nk < 1 || nk > kPkMaxBins) return hipErrorInvalidValue;
+  hipError_t e = hipMemsetAsync(count, 0, sizeof(unsigned long long) * nk, s);
+  if (e == hipSuccess) e = hipMemsetAsync(sums, 0, sizeof(double) * 3 * nk, s);
+  if (e != hipSuccess) return e;
+  switch (N) {
+#define X(NN) \
+  case NN:    \
+    return pk_impl<NN>(S, D0, DN, dk, lnkmin, dlnk, nk, pscale, count, sums, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

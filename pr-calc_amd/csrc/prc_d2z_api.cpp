@@ -113,6 +113,7 @@ struct prc_d2z_ctx {
   double* d_Rint = nullptr;
   float* d_R = nullptr;
   unsigned* d_hist = nullptr;
+  double* d_pk = nullptr;  // power spectrum bins: counts (uint64) then 3 sums, kPkMaxBins each
   int* d_flag = nullptr;
   SetupScalars sc{};
   std::vector<RadiusConst> rc;
@@ -540,6 +541,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_Rint, sizeof(double) * (NS + 2));
   A_(c->d_R, sizeof(float) * NS);
   A_(c->d_hist, sizeof(unsigned) * kNHistory);
+  A_(c->d_pk, sizeof(double) * 4 * kPkMaxBins);
   A_(c->d_flag, sizeof(int));
   A_(c->d_grfA, sizeof(double));
 #undef A_
@@ -802,6 +804,53 @@ int prc_d2z_displacement(prc_d2z_ctx* c, int axis, float* out, int layout) {
   float* real = reinterpret_cast<float*>(T1);
   PRC_CHECK(c, launch_inv_z(N, T2, real, c->d_st_half, c->d_tw_full, s), "inverse z pass");
   return copy_field_out(c, real, out, layout);
+}
+
+// PowerSpectrum (powerspectrum_lmb.C:8-139) of the context's density field:
+// the set-up in the reference's float arithmetic (dk, kmax, d3k, dlnk), the
+// binning on the GPU, the per-bin normalisations as the reference's
+// (:108-110), NaN for empty bins as its divisions by zero give.
+int prc_d2z_power_spectrum(prc_d2z_ctx* c, int nk, float kmin, float kmax, double* ps, double* ook2, double* kmean,
+                           int64_t* nmodes) {
+  if (!c || !ps || !ook2 || !kmean || !nmodes) return PRC_D2Z_ERR_BAD_ARG;
+  if (nk < 1 || nk > kPkMaxBins)
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "nk must be in 1.." + std::to_string(kPkMaxBins));
+  if (!(kmin > 0.0f) || !(kmax > kmin)) return c->fail(PRC_D2Z_ERR_BAD_ARG, "need 0 < kmin < kmax");
+  if (!c->have_delta) return c->fail(PRC_D2Z_ERR_STATE, "no density field: call prc_d2z_set_delta first");
+  (void)hipSetDevice(c->p.device);
+  const bool was_timing = c->timing;
+  c->timing = false;
+  int rc = c->have_spectrum ? PRC_D2Z_OK : forward(c, c->sc.DInit);
+  c->timing = was_timing;
+  if (rc) return rc;
+  const int N = c->N;
+  const float dk = (float)(2. * M_PI / (double)c->p.BoxSize);  // powerspectrum_lmb.C:18
+  const float kmaxg = dk * N;                                    // :19
+  const float d3k = (float)pow((double)kmaxg, 3);                // :20
+  const float lnkmin = logf(kmin);
+  const float dlnk = (logf(kmax) - logf(kmin)) / nk;              // :26
+  // |FFT(delta) N^-1.5|^2 / d3k from S = FFT(delta / DInit)
+  const double pscale = (double)c->sc.DInit * c->sc.DInit / ((double)N * N * N) / (double)d3k;
+  unsigned long long* d_count = reinterpret_cast<unsigned long long*>(c->d_pk);
+  double* d_sums = c->d_pk + kPkMaxBins;
+  PRC_CHECK(c, launch_pk(N, c->d_S, c->d_D0, c->d_DN, dk, lnkmin, dlnk, nk, pscale, d_count, d_sums, c->stream),
+            "power spectrum");
+  std::vector<unsigned long long> cnt(nk);
+  std::vector<double> sums(3 * (size_t)nk);
+  PRC_CHECK(c, hipMemcpyAsync(cnt.data(), d_count, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, c->stream),
+            "power spectrum copy");
+  PRC_CHECK(c, hipMemcpyAsync(sums.data(), d_sums, sizeof(double) * 3 * nk, hipMemcpyDeviceToHost, c->stream),
+            "power spectrum copy");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "power spectrum copy");
+  const double twopi3 = pow(2 * M_PI, 3);
+  for (int b = 0; b < nk; ++b) {
+    const double n = (double)cnt[b];
+    nmodes[b] = (int64_t)cnt[b];
+    ook2[b] = sums[nk + b] / n;
+    kmean[b] = sums[2 * nk + b] / n;
+    ps[b] = sums[b] * twopi3 / n;
+  }
+  return PRC_D2Z_OK;
 }
 
 int prc_d2z_set_timing(prc_d2z_ctx* c, int enable) {

@@ -69,7 +69,7 @@ EXPORTS = [
     "prc_d2z_last_error", "prc_d2z_set_delta", "prc_d2z_set_delta_device", "prc_d2z_run",
     "prc_d2z_run_async", "prc_d2z_sync", "prc_d2z_get_zreion", "prc_d2z_get_rbubble",
     "prc_d2z_get_scalars", "prc_d2z_get_radius_table", "prc_d2z_zreion_device", "prc_d2z_history",
-    "prc_d2z_smoothed_field", "prc_d2z_displacement", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
+    "prc_d2z_smoothed_field", "prc_d2z_displacement", "prc_d2z_power_spectrum", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
 ]
@@ -88,6 +88,7 @@ def load_library(path: str | os.PathLike | None = None):
     lib = C.CDLL(str(p))
     vp = C.c_void_p
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
     up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
     sig = {
         "prc_d2z_abi_version": (C.c_int, []),
@@ -108,6 +109,8 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_history": (C.c_int, [vp, up]),
         "prc_d2z_smoothed_field": (C.c_int, [vp, C.c_int, fp]),
         "prc_d2z_displacement": (C.c_int, [vp, C.c_int, fp, C.c_int]),
+        "prc_d2z_power_spectrum": (C.c_int, [vp, C.c_int, C.c_float, C.c_float, dp, dp, dp,
+                                             np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
         "prc_d2z_set_timing": (C.c_int, [vp, C.c_int]),
         "prc_d2z_get_timing": (C.c_int, [vp, C.POINTER(CTiming)]),
         "prc_d2z_device_bytes": (C.c_size_t, [vp]),
@@ -254,6 +257,14 @@ class D2Z:
         out = np.empty((N, N, N + 2) if layout == PADDED else (N, N, N), dtype=np.float32)
         self._check(self.lib.prc_d2z_displacement(self._ctx, axis, out, layout))
         return out
+
+    def power_spectrum(self, nk: int, kmin: float, kmax: float) -> dict:
+        """Binned power spectrum of the density field, as PowerSpectrum
+        (powerspectrum_lmb.C:8-139): arrays ps, ook2, kmean, nmodes."""
+        ps, ook2, kmean = (np.empty(nk, dtype=np.float64) for _ in range(3))
+        nmodes = np.empty(nk, dtype=np.int64)
+        self._check(self.lib.prc_d2z_power_spectrum(self._ctx, nk, kmin, kmax, ps, ook2, kmean, nmodes))
+        return {"ps": ps, "ook2": ook2, "kmean": kmean, "nmodes": nmodes}
 
     def set_launch_config(self, radii_per_launch: int, z_blocks: int):
         self._check(self.lib.prc_d2z_set_launch_config(self._ctx, radii_per_launch, z_blocks))

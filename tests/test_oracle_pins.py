@@ -364,3 +364,45 @@ def test_displacement_zeroes_the_components_own_nyquist_plane():
     ref_y = -(ky / (kx * kx + ky * ky)) * np.sin(ph)
     assert np.max(np.abs(o.displacement(delta, box, 1) - ref_y)) <= 1e-12 * np.max(np.abs(ref_y))
     assert np.max(np.abs(o.displacement(delta, box, 2))) <= 1e-14
+
+
+# ------------------------------------------------------------ PowerSpectrum
+def test_power_spectrum_of_a_plane_wave_is_closed_form():
+    """cos(k0.x) puts |delta_k|^2 = (N^3/2 N^-1.5)^2 = N^3/4 into the one mode
+    +k0 of the half spectrum: ps of its bin = (2 pi)^3 (N^3/4) / d3k / count."""
+    N, box = 16, 30.0
+    n0 = (3, 1, 2)
+    delta = np.cos(_plane(N, n0))
+    dk = 2.0 * np.pi / box
+    k0 = dk * np.sqrt(np.dot(n0, n0))
+    nk, kmin, kmax = 12, 0.93 * dk, 0.47 * dk * N * np.sqrt(3.0)
+    r = o.power_spectrum(delta, box, nk, kmin, kmax)
+    b0 = int((np.log(k0) - np.log(kmin)) / ((np.log(kmax) - np.log(kmin)) / nk))
+    d3k = np.float64(np.float32(np.float64(np.float32(np.float32(dk) * N)) ** 3))
+    assert r["ps"][b0] == pytest.approx((2 * np.pi) ** 3 * (N ** 3 / 4.0) / d3k / r["nmodes"][b0], rel=1e-12)
+    others = [b for b in range(nk) if b != b0 and r["nmodes"][b] > 0]
+    assert max(r["ps"][others]) <= 1e-20 * r["ps"][b0]
+
+
+def test_power_spectrum_mode_counts_brute_force():
+    """Counts of the half-spectrum modes per log bin against an independent
+    FP64 lattice count (the bins chosen so no mode sits within 1e-4 of an
+    edge), including the reference's truncation of u in (-1, 0) into bin 0."""
+    N, box, nk = 16, 25.0, 9
+    dk = 2.0 * np.pi / box
+    kmin, kmax = 1.37 * dk, 6.91 * dk
+    r = o.power_spectrum(np.random.default_rng(0).standard_normal((N, N, N)), box, nk, kmin, kmax)
+    dl = (np.log(kmax) - np.log(kmin)) / nk
+    counts = np.zeros(nk, dtype=np.int64)
+    for nx in range(-N // 2, N // 2):
+        for ny in range(-N // 2, N // 2):
+            for nz in range(0, N // 2 + 1):
+                nzs = -N // 2 if nz == N // 2 else nz
+                k = dk * np.sqrt(nx * nx + ny * ny + nzs * nzs)
+                if k == 0:
+                    continue
+                u = (np.log(k) - np.log(kmin)) / dl
+                assert abs(u - round(u)) > 1e-4
+                if -1.0 < u < nk:
+                    counts[int(u)] += 1
+    assert np.array_equal(r["nmodes"], counts)
