@@ -68,7 +68,7 @@ typedef struct {
   float Rmax, zeta, M_min;
   /* execution options (no counterpart in the reference) */
   int mode;             /* enum prc_d2z_mode */
-  int radii_per_launch; /* fused modes: radii per batch, 1..16; 0 = default */
+  int radii_per_launch; /* fused modes: radii per batch, 1..32; 0 = default */
   int device;           /* HIP device ordinal */
 } prc_d2z_params;
 

@@ -35,7 +35,8 @@ struct GrfSpectrum {
   double invN3;  // 1/N^3 of the inverse transform
 };
 
-constexpr int kFusedMaxRadii = 16;  // radii per fused batch
+constexpr int kFusedMaxRadii = 32;  // radii per fused batch (y and z passes)
+constexpr int kFxMaxRadii = 16;     // radii per x-pass launch (a batch takes one or more)
 
 // Strides of one tiled pass (complex elements); see pass_kernels.hip.
 struct TileMap {

@@ -508,7 +508,7 @@ bool fx_mfma_supported(int N, bool half) { return half && N == kN; }
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
                           const WTable& wt, const void* mats, hipStream_t s) {
-  if (N != kN || nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax || !mats) return hipErrorInvalidValue;
+  if (N != kN || nb < 1 || nb > kFxMaxRadii || wt.M < 0 || wt.M > kWTableMax || !mats) return hipErrorInvalidValue;
   const long long nblk = (long long)kN * ((kN / 2) / kC) * kNSUB;
   hipLaunchKernelGGL(k_fx_mfma, dim3((unsigned)nblk), dim3(kThreads), sizeof(float4) * wt.M, s, S, D0, DN,
                      static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan, wt.coef, wt.M, wt.invh,

@@ -1022,7 +1022,7 @@ hipError_t launch_disp(int N, const float2* S, const float2* D0, const float2* D
 hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, bool half, float dk,
                      float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, const WTable& wt,
                      hipStream_t s) {
-  if (nb < 1 || nb > kFusedMaxRadii || wt.M < 0 || wt.M > kWTableMax) return hipErrorInvalidValue;
+  if (nb < 1 || nb > kFxMaxRadii || wt.M < 0 || wt.M > kWTableMax) return hipErrorInvalidValue;
   switch (N) {
 #define X(NN)                                                                                                  \
   case NN:                                                                                                     \

@@ -249,20 +249,27 @@ MixedScales mixed_scales(int N) {
   return {ldexpf(1.0f, e1), ldexpf(1.0f, -e2), ldexpf(1.0f, e2 - e1)};
 }
 
-// The x pass of one batch (filter + x inverse) into the intermediates T, on s
+// The x pass of one batch (filter + x inverse) into the intermediates T, on s:
+// launches of at most kFxMaxRadii radii (a launch's blocks write one plane per
+// radius; 32 planes per block measured slower), each into its planes of T
 int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
   const int N = c->N;
   const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
   const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
   const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
-  PRC_CHECK(c, timed(c, "fx_filter", [&] {
-              if (fx_mfma_supported(N, half))
-                return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, T, host_dk(c), invN3, c->d_rc, r0, nb,
-                                      c->d_st_full, c->wt, c->d_mats, s);
-              return launch_fx(N, c->d_S, c->d_D0, c->d_DN, T, half, host_dk(c), invN3, c->d_rc, r0, nb,
-                               c->d_st_fx, c->wt, s);
-            }),
-            "fused x pass");
+  const size_t plane_bytes = (half ? sizeof(__half2) : sizeof(float2)) * (size_t)c->nspec;
+  for (int off = 0; off < nb; off += kFxMaxRadii) {
+    const int n = std::min(kFxMaxRadii, nb - off);
+    void* Tc = reinterpret_cast<char*>(T) + plane_bytes * off;
+    PRC_CHECK(c, timed(c, "fx_filter", [&] {
+                if (fx_mfma_supported(N, half))
+                  return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, Tc, host_dk(c), invN3, c->d_rc, r0 + off, n,
+                                        c->d_st_full, c->wt, c->d_mats, s);
+                return launch_fx(N, c->d_S, c->d_D0, c->d_DN, Tc, half, host_dk(c), invN3, c->d_rc, r0 + off, n,
+                                 c->d_st_fx, c->wt, s);
+              }),
+              "fused x pass");
+  }
   return PRC_D2Z_OK;
 }
 
@@ -504,7 +511,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   size_t freeb = 0, totalb = 0;
   const bool have_info = hipMemGetInfo(&freeb, &totalb) == hipSuccess;
   auto tbytes_for = [&](int nb) { return std::max((size_t)2 * sizeof(float2) * c->nspec, tstore * c->nspec * nb); };
-  // default batch: the largest radii-per-launch (<= 16) whose intermediates fit
+  // default batch: the largest radii-per-launch (<= 32) whose intermediates fit
   if (p->mode != PRC_D2Z_PLAIN_FP32 && !p->radii_per_launch && have_info)
     while (c->nbatch > 1 && fixed + tbytes_for(c->nbatch) > freeb) --c->nbatch;
   c->tbytes = tbytes_for(c->nbatch);

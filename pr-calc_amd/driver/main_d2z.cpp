@@ -73,7 +73,7 @@ void usage() {
   printf("\n   -v print stdout to the terminal instead of the .stdout file");
   printf("\n   -e print stderr to the terminal instead of the .stderr file");
   printf("\n   -M plain|fused|mixed  GPU execution path [default = fused]");
-  printf("\n   -b <n> radii per fused launch, 1..16 [default = 16]");
+  printf("\n   -b <n> radii per fused batch, 1..32 [default = 32]");
   printf("\n   -d <n> HIP device [default = 0]");
   printf("\n\n");
   exit(0);

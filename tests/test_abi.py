@@ -55,7 +55,7 @@ def test_unsupported_n_rejected_before_device(N):
 
 
 def test_bad_arguments_rejected():
-    for kw in ({"Nscales": 1}, {"h": 0.0}, {"zeta": 1.0}, {"radii_per_launch": 17}, {"mode": 7}):
+    for kw in ({"Nscales": 1}, {"h": 0.0}, {"zeta": 1.0}, {"radii_per_launch": 33}, {"mode": 7}):
         with pytest.raises(D2ZError) as ei:
             D2Z(D2ZParams(N=32, **kw))
         assert ei.value.code == -1, kw

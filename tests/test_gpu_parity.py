@@ -82,8 +82,10 @@ def test_zreion_vs_oracle(N, mode):
 def test_fused_matches_plain_and_batching():
     N = 64
     outs = {}
-    # 20 radii: batches of 16 + 4, 8 + 8 + 4, 3 x 6 + 2 and single radii
-    for mode, rpl in [(PLAIN_FP32, 0), (FUSED_FP32, 1), (FUSED_FP32, 3), (FUSED_FP32, 8), (FUSED_FP32, 16)]:
+    # 20 radii: one batch (x pass in launches of 16 + 4), batches of 16 + 4,
+    # 8 + 8 + 4, 3 x 6 + 2 and single radii
+    for mode, rpl in [(PLAIN_FP32, 0), (FUSED_FP32, 1), (FUSED_FP32, 3), (FUSED_FP32, 8), (FUSED_FP32, 16),
+                      (FUSED_FP32, 0)]:
         announce(f"fused vs plain N={N} mode={mode} rpl={rpl}")
         with D2Z(lib_params(N, nscales=20, mode=mode, rpl=rpl)) as d:
             d.set_delta(field(N))
@@ -96,6 +98,7 @@ def test_fused_matches_plain_and_batching():
     assert np.array_equal(outs[(FUSED_FP32, 1)], outs[(FUSED_FP32, 8)])
     assert np.array_equal(outs[(FUSED_FP32, 3)], outs[(FUSED_FP32, 8)])
     assert np.array_equal(outs[(FUSED_FP32, 16)], outs[(FUSED_FP32, 8)])
+    assert np.array_equal(outs[(FUSED_FP32, 0)], outs[(FUSED_FP32, 8)])
 
 
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32])
