@@ -50,6 +50,9 @@ DTYPES = {"plain": "f32", "fused": "f32",
           "mixed": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector and barrier"}
 
 
+FX_MAX_RADII = 16  # radii per fused x-pass launch (the library's launch limit)
+
+
 def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:
     """Algorithmic HBM bytes one launch of a kernel must move.
 
@@ -159,11 +162,15 @@ def main():
     nb = d.launch_config()["radii_per_launch"] if args.mode != "plain" else 1
     store = 2 if args.mode == "mixed" else 4
     # algorithmic bytes of the dominant kernel over the step (the last fused
-    # launch may carry fewer radii), per launch on average
+    # batch may carry fewer radii; the x pass fills a batch in launches of at
+    # most FX_MAX_RADII, each reading the spectrum), per launch on average
     if args.mode == "plain" or dom_name not in ("fx_filter", "inv_y", "fz_barrier"):
         alg_step = kernel_alg_bytes(dom_name, N, 1, store) * dom_n
     else:
-        alg_step = sum(kernel_alg_bytes(dom_name, N, min(nb, NS - r0), store) for r0 in range(0, NS, nb))
+        launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
+        if dom_name == "fx_filter":
+            launches = [min(FX_MAX_RADII, n - o) for n in launches for o in range(0, n, FX_MAX_RADII)]
+        alg_step = sum(kernel_alg_bytes(dom_name, N, n, store) for n in launches)
     alg = alg_step / dom_n
     achieved = alg_step / (dom_ms * 1e-3) / 1e9 if alg_step else None
     roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -206,7 +213,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{N}^3 box {box:g} Mpc/h, {NS} radii 1 cell..{rmax:g} Mpc/h, "
                                f"zeta 40, Mmin 1e8 Msun/h, GRF seed 1", "mode": args.mode,
-                   "radii_per_launch": nb},
+                   "radii_per_batch": nb, "x_pass_radii_per_launch": min(nb, FX_MAX_RADII)},
         "graph_ms_per_step": graph_ms,
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
