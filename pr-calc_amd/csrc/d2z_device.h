@@ -1,7 +1,9 @@
 // Device helpers shared by the pass kernels: blocked layout, XCD tile
 // mapping, the top-hat filter, the barrier and FP64 wave reductions.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 #include <math.h>
 
 #include "d2z_kernels.h"
@@ -163,6 +165,39 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// ------------------------------------------------------ FP16 complex pairs
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_h2(float re, float im) {
+  const __half2 h = __floats2half2_rn(re, im);
+  return *reinterpret_cast<const uint32_t*>(&h);
+}
+__device__ __forceinline__ float2 unpack_h2(uint32_t u) {
+  __half2 h;
+  *reinterpret_cast<uint32_t*>(&h) = u;
+  return __half22float2(h);
+}
+
+// Real-output glue of the half-length c2r in packed FP16: Z[k] = P a + Q conj(m)
+// with a = X[k], m = X[(L - k) mod L], P = 1 + i w^k, Q = 1 - i w^k (w = exp(2 pi
+// i / N)), the same as (a + conj m) + i w^k (a - conj m); P = 0, Q = 1 + i at
+// k = 0 give the packed slot's (a.x + a.y, a.x - a.y).  Four packed FMAs, the
+// swaps and signs in their operand modifiers.
+__device__ __forceinline__ uint32_t c2r_glue_h2(uint32_t av, uint32_t mv, uint32_t pv, uint32_t qv) {
+  const half2_t a = __builtin_bit_cast(half2_t, av), m = __builtin_bit_cast(half2_t, mv);
+  const half2_t P = __builtin_bit_cast(half2_t, pv), Q = __builtin_bit_cast(half2_t, qv);
+  half2_t z = a * P.xx;                                 // (P.x a.x, P.x a.y)
+  z = __builtin_elementwise_fma(a.yx, (half2_t){-P.y, P.y}, z);  // (-P.y a.y, P.y a.x)
+  z = __builtin_elementwise_fma(m, (half2_t){Q.x, -Q.x}, z);     // (Q.x m.x, -Q.x m.y)
+  z = __builtin_elementwise_fma(m.yx, Q.yy, z);                  // (Q.y m.y, Q.y m.x)
+  return __builtin_bit_cast(uint32_t, z);
+}
+// The glue constants (P, Q) of c2r_glue_h2 for k, from w^k = exp(2 pi i k/N)
+__device__ __forceinline__ uint2 c2r_glue_pq(int k, float2 wk) {
+  return k == 0 ? make_uint2(pack_h2(0.f, 0.f), pack_h2(1.f, 1.f))
+                : make_uint2(pack_h2(1.f - wk.y, wk.x), pack_h2(1.f + wk.y, -wk.x));
 }
 
 }  // namespace prc

@@ -417,4 +417,18 @@ __device__ __forceinline__ void mirror_lanes(const float2 (&v)[P::E], float2 (&m
   }
 }
 
+// The same for 32-bit words (FP16 complex pairs): one move per register
+template <class P>
+__device__ __forceinline__ void mirror_lanes(const uint32_t (&v)[P::E], uint32_t (&m)[P::E], int i) {
+  constexpr int E = P::E, T = P::T;
+  static_assert(T <= 64 && 64 % T == 0, "pencil must fit in one wave");
+  const int lane = __lane_id();
+  const int src = (lane - i) + ((T - i) % T);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t got = (uint32_t)__shfl((int)v[E - 1 - e], src, 64);
+    m[e] = (i == 0) ? v[(E - e) % E] : got;
+  }
+}
+
 }  // namespace prc

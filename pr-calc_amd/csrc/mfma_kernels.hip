@@ -31,7 +31,6 @@ namespace prc {
 namespace {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kN = 1024;         // transform length and grid size
@@ -68,16 +67,6 @@ __device__ half8_t stage_matrix(int k, int p, int h, int l) {
 // every access pattern of the three stages over all 32 banks.
 __device__ __forceinline__ int xrow(int pos) { return pos ^ (((pos >> 8) & 1) | (((pos >> 4) & 1) << 1)); }
 
-__device__ __forceinline__ uint32_t pack_h2(float re, float im) {
-  const __half2 h = __floats2half2_rn(re, im);
-  return *reinterpret_cast<const uint32_t*>(&h);
-}
-__device__ __forceinline__ float2 unpack_h2(uint32_t u) {
-  __half2 h;
-  *reinterpret_cast<uint32_t*>(&h) = u;
-  return __half22float2(h);
-}
-
 // B operand from 4 complex values
 __device__ __forceinline__ half8_t pack_b(const float2 (&x)[4]) {
   half8_t b;
@@ -113,21 +102,6 @@ __device__ __forceinline__ void store_d(uint32_t* buf, const float4_t& d0, const
 constexpr int kMatSets = 2 + 2 * 16;
 __device__ __forceinline__ int mat_set_a(int h) { return h; }
 __device__ __forceinline__ int mat_set_b(int k, int h) { return 2 + 2 * k + h; }
-
-// Real-output glue of the half-length c2r in packed FP16: Z[k] = P a + Q conj(m)
-// with a = X[k], m = X[(L - k) mod L], P = 1 + i w^k, Q = 1 - i w^k (w = exp(2 pi
-// i / N)), the same as (a + conj m) + i w^k (a - conj m); P = 0, Q = 1 + i at
-// k = 0 give the packed slot's (a.x + a.y, a.x - a.y).  Four packed FMAs, the
-// swaps and signs in their operand modifiers.
-__device__ __forceinline__ uint32_t c2r_glue_h2(uint32_t av, uint32_t mv, uint32_t pv, uint32_t qv) {
-  const half2_t a = __builtin_bit_cast(half2_t, av), m = __builtin_bit_cast(half2_t, mv);
-  const half2_t P = __builtin_bit_cast(half2_t, pv), Q = __builtin_bit_cast(half2_t, qv);
-  half2_t z = a * P.xx;                                 // (P.x a.x, P.x a.y)
-  z = __builtin_elementwise_fma(a.yx, (half2_t){-P.y, P.y}, z);  // (-P.y a.y, P.y a.x)
-  z = __builtin_elementwise_fma(m, (half2_t){Q.x, -Q.x}, z);     // (Q.x m.x, -Q.x m.y)
-  z = __builtin_elementwise_fma(m.yx, Q.yy, z);                  // (Q.y m.y, Q.y m.x)
-  return __builtin_bit_cast(uint32_t, z);
-}
 
 __global__ void __launch_bounds__(64) k_mfma_mats(half8_t* __restrict__ out) {
   const int set = blockIdx.x, l = threadIdx.x;
@@ -311,9 +285,7 @@ __global__ void __launch_bounds__(512)
   const int rowA = n & 7, sel = n >> 3;  // MFMA column n = (row, butterfly sel)
   if (tid < kRowsZ * kFusedMaxRadii) red[tid / kFusedMaxRadii][tid % kFusedMaxRadii] = 0.0;
   for (int q = tid; q < kLz; q += 512) {
-    const float2 w = tw_full[q];
-    s_pq[q] = q == 0 ? make_uint2(pack_h2(0.f, 0.f), pack_h2(1.f, 1.f))
-                     : make_uint2(pack_h2(1.f - w.y, w.x), pack_h2(1.f + w.y, -w.x));
+    s_pq[q] = c2r_glue_pq(q, tw_full[q]);
   }
   for (int q = tid; q < kLz / 2; q += 512) s_tw2[q] = tw_full[2 * q];
   if (w < 2) s_gA[w][l] = mats[mat_set_a(w) * 64 + l];

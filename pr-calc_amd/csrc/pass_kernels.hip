@@ -507,14 +507,23 @@ __global__ void __launch_bounds__(256)
   // twiddles in LDS (the blocks are persistent: copied once), the plan's
   // stages transposed so that lanes with consecutive k read consecutive words
   constexpr int NTW = plan_tw_size<P>() > 0 ? plan_tw_size<P>() : 1;
+  // c2r glue: w^k for FP32 storage, the packed FP16 constants (P, Q) of
+  // c2r_glue_h2 for FP16 storage
+  constexpr bool kH = std::is_same<TS, __half2>::value;
   __shared__ float2 s_twh[NTW];
   __shared__ float2 s_twf[L];
+  uint2* const s_pq = reinterpret_cast<uint2*>(s_twf);
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
   const int w = t >> 6, lane = t & 63;
   if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
   copy_twiddles_transposed<P>(s_twh, tw_half, t, 256);
-  for (int q = t; q < L; q += 256) s_twf[q] = tw_full[q];
+  for (int q = t; q < L; q += 256) {
+    if constexpr (kH)
+      s_pq[q] = c2r_glue_pq(q, tw_full[q]);
+    else
+      s_twf[q] = tw_full[q];
+  }
   __syncthreads();
   const long long plane = nrows * L;
   // FP16 storage with 8-element pencils (N <= 1024): the pass is issue-bound
@@ -522,7 +531,7 @@ __global__ void __launch_bounds__(256)
   // the waves' loads (adjacent segments of each kz block) together in time:
   // wave-level sync measured 8% slower for FP32 at N = 1024 and 40% slower
   // for FP16 at N = 2048.
-  using Lds = LdsRow<L, std::is_same<TS, __half2>::value && (E <= 8)>;
+  using Lds = LdsRow<L, kH && (E <= 8)>;
   const Lds lds{smem + rr * Lds::STRIDE};
   const double sc = (double)iscale, sc2 = sc * sc;
 #pragma unroll 1
@@ -541,14 +550,28 @@ __global__ void __launch_bounds__(256)
 #pragma unroll 1
     for (int q = 0; q < nb; ++q) {
       const TS* tp = Tbuf + (long long)q * plane;
-      float2 v[E], m[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+      float2 v[E];
       // read per radius from LDS rather than held in registers across radii
       int zero = 0;
       asm volatile("" : "+s"(zero));
-      mirror_lanes<P>(v, m, i);
-      c2r_pre<P>(v, m, i, s_twf + zero);
+      if constexpr (kH) {  // glue on the stored FP16 pairs: one move and four packed FMAs each
+        const uint32_t* tu = reinterpret_cast<const uint32_t*>(tp);
+        uint32_t a[E], m[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[e] = tu[zrow_offset<P, C>(i, e, r, nrows)];
+        mirror_lanes<P>(a, m, i);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint2 pq = s_pq[zero + i + e * T];
+          v[e] = unpack_h2(c2r_glue_h2(a[e], m[e], pq.x, pq.y));
+        }
+      } else {
+        float2 m[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+        mirror_lanes<P>(v, m, i);
+        c2r_pre<P>(v, m, i, s_twf + zero);
+      }
       pencil_fft<P, +1, 0, Lds, true>(v, i, s_twh + zero, lds);
       // the output scale iscale (a power of two) is applied inside the
       // barrier's FMA and to the sum of squares, where it is exact
@@ -559,9 +582,8 @@ __global__ void __launch_bounds__(256)
       float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        constexpr bool kF32Barrier = std::is_same<TS, __half2>::value;
-        const float z0 = kF32Barrier ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
-        const float z1 = kF32Barrier ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
+        const float z0 = kH ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
+        const float z1 = kH ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
         acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
         if constexpr (RB) {
           if (z0 > best[e].x) {
