@@ -29,6 +29,10 @@ extern "C" {
 
 #define PRC_D2Z_ABI_VERSION 1
 #define PRC_D2Z_NHISTORY 100000 /* NHISTORY, c++/src/allvars_d2z.h:1 */
+#define PRC_D2Z_NCHISTORY 10000 /* NCHISTORY, c++/src/allvars_d2z.h:5 */
+#define PRC_D2Z_NX_MFP 1000     /* NX_MFP, c++/src/allvars_d2z.h:9 */
+#define PRC_D2Z_NMFP 200        /* NMFP, c++/src/allvars_d2z.h:13 */
+#define PRC_D2Z_NSKEWERS 20000  /* NSKEWERS, c++/src/allvars_d2z.h:17 */
 
 typedef struct prc_d2z_ctx prc_d2z_ctx;
 
@@ -145,6 +149,39 @@ int prc_d2z_zreion_device(prc_d2z_ctx* ctx, const float** d_zreion);
 /* GetHistory (c++/src/bubblehistory_d2z.C:5-41) on the GPU: integer counts per
  * history bin, bin = (int)((zreion - 49.9)/dzhistory). */
 int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */);
+
+/* History correction of the written zreion field (main_d2z.C:78-90), in the
+ * reference's order: GetHistory (above, history = cumulative counts / N^3 as
+ * float), GetBubbleMFP, GetCHistory, CorrectHistory.
+ *
+ * prc_d2z_bubble_mfp -- GetBubbleMFP (c++/src/bubblehistory_d2z.C:82-242) on
+ * the GPU for one rank: for each ionized fraction x_i = 0.01 + i dx (i <
+ * PRC_D2Z_NX_MFP, dx = 0.989/999) and zcur_i where the float history
+ * [PRC_D2Z_NHISTORY] crosses x_i, nskewers skewers (0 = PRC_D2Z_NSKEWERS) start
+ * at random cells with zreion > zcur_i (j, k < N - 2), walk along +y and +z to
+ * the first cell with zreion < zcur_i, and record a distance log-uniform in
+ * [m - 1, m] cells.  Random numbers: Philox4x32-10 keyed by seed, counter
+ * (skewer, i, draw) -- the reference's rand() sequence is not reproduced.
+ * mfp[i] = mean distance (BoxSize/N units of the parameter file), mfp_x[i] =
+ * x_i, dfdmfp[i * PRC_D2Z_NMFP + b] = walks in ln-distance bin b between
+ * ln(BoxSize/N) and ln BoxSize (mfp_x, dfdmfp nullable).  A skewer that finds
+ * no start cell in 16384 draws fails the call (the reference loops forever). */
+int prc_d2z_bubble_mfp(prc_d2z_ctx* ctx, const float* history, int nskewers, uint64_t seed, float* mfp,
+                       float* mfp_x, float* dfdmfp);
+/* prc_d2z_corrected_history -- GetCHistory (c++/src/bubblehistory_d2z.C:44-80)
+ * on the host: chistory[PRC_D2Z_NCHISTORY] from the collapsed-fraction table
+ * fcoll(z) (the file fcoll_table.tab: float min, float max, int n, float
+ * table[n]; Float2FloatTable, c++/src/tables.C:9-38), the mean free paths of
+ * prc_d2z_bubble_mfp and the context's Rmax and zeta. */
+int prc_d2z_corrected_history(prc_d2z_ctx* ctx, const float* fcoll, int nfcoll, float fmin, float fmax,
+                              const float* mfp, float* chistory);
+/* prc_d2z_correct_zreion -- CorrectHistory (c++/src/bubblehistory_d2z.C:270-343)
+ * on the GPU: every cell's zreion becomes the redshift at which the corrected
+ * history reaches the history's value in the cell's bin; bins are clamped to
+ * the table (the reference reads out of bounds for zreion < 0.01 or > 49.9).
+ * Replaces the context's zreion (prc_d2z_get_zreion returns the corrected
+ * field). */
+int prc_d2z_correct_zreion(prc_d2z_ctx* ctx, const float* history, const float* chistory);
 
 /* Smoothed field delta_R of radius r (filter + c2r + 1/N^3), dense layout,
  * through the context's transform path. */

@@ -46,7 +46,7 @@ def build_library(jobs: int = 4) -> Path:
     objdir = LIB / "obj"
     objdir.mkdir(exist_ok=True)
     srcs = [CSRC / "d2z_kernels.hip", CSRC / "pass_kernels.hip", CSRC / "mfma_kernels.hip", CSRC / "grf_kernels.hip",
-            CSRC / "prc_d2z_api.cpp"]
+            CSRC / "bubble_kernels.hip", CSRC / "prc_d2z_api.cpp"]
     objs = []
     procs = []
     for s in srcs:

@@ -5,6 +5,23 @@
 namespace prc {
 
 constexpr int kNHistory = 100000;  // NHISTORY, c++/src/allvars_d2z.h:1
+constexpr int kNCHistory = 10000;  // NCHISTORY, c++/src/allvars_d2z.h:5
+constexpr int kNXMfp = 1000;       // NX_MFP, c++/src/allvars_d2z.h:9
+constexpr int kNMfp = 200;         // NMFP, c++/src/allvars_d2z.h:13
+constexpr int kNSkewers = 20000;   // NSKEWERS, c++/src/allvars_d2z.h:17
+// Start-cell draws per skewer before the call fails (the reference draws until
+// it finds an ionized cell, without bound; at an ionized fraction of 1% the
+// chance of 16384 misses is e^-164)
+constexpr int kMaxStartDraws = 16384;
+
+// Skewer walk set-up: grid size, skewers per ionized fraction, Philox key, cell
+// size (BoxSize/N as float, widened) and the log-distance bins of the histogram
+struct SkewerParams {
+  int N, nskewers;
+  unsigned int seed_lo, seed_hi;
+  double cell;
+  float lmin, lmax, dmfp;
+};
 
 // Reference parameters as the reference stores them (floats; BoxSize in
 // Mpc/h as read from the parameter file, Rmax in Mpc/h, M_min in Msun).
@@ -126,6 +143,11 @@ hipError_t launch_moments(const float* x, long long total, double2* partial, int
 hipError_t launch_moments_finish(const double2* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
                                  int r0, hipStream_t s);
 hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, hipStream_t s);
+// history correction (bubble_kernels.hip)
+hipError_t launch_skewers(const float* zre, const float* zcur, const SkewerParams& p, double* sums,
+                          unsigned int* counts, int* fail, hipStream_t s);
+hipError_t launch_zvz(const float* history, const float* chist, float dzch, float* zvz, hipStream_t s);
+hipError_t launch_remap(float* zre, long long n, const float* zvz, double dz, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, double* integ, float* Rr, double* Rint, SetupScalars* sc,
                         RadiusConst* rc, float* sigma_out, hipStream_t s);
 

@@ -21,24 +21,6 @@
 
 namespace prc {
 
-__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    if (r) {
-      k.x += W0;
-      k.y += W1;
-    }
-    const uint64_t p0 = (uint64_t)M0 * c.x;
-    const uint64_t p1 = (uint64_t)M1 * c.z;
-    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
-    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-  }
-  return c;
-}
-
 __global__ void k_white_noise(float* __restrict__ out, long long n, unsigned long long seed) {
   const uint2 key = make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
   for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;

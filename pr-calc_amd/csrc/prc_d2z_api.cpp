@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <string>
 #include <vector>
@@ -115,6 +116,14 @@ struct prc_d2z_ctx {
   unsigned* d_hist = nullptr;
   double* d_pk = nullptr;  // power spectrum bins: counts (uint64) then 3 sums, kPkMaxBins each
   int* d_flag = nullptr;
+  // history correction: skewer sums and counts per ionized fraction, zcur per
+  // fraction, the history in, the corrected history and the per-bin table
+  double* d_mfp_sum = nullptr;
+  unsigned* d_mfp_cnt = nullptr;
+  float* d_zcur = nullptr;
+  float* d_hin = nullptr;
+  float* d_chist = nullptr;
+  float* d_zvz = nullptr;
   SetupScalars sc{};
   std::vector<RadiusConst> rc;
   bool have_delta = false, have_spectrum = false, have_result = false;
@@ -550,6 +559,12 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_hist, sizeof(unsigned) * kNHistory);
   A_(c->d_pk, sizeof(double) * 4 * kPkMaxBins);
   A_(c->d_flag, sizeof(int));
+  A_(c->d_mfp_sum, sizeof(double) * kNXMfp);
+  A_(c->d_mfp_cnt, sizeof(unsigned) * kNXMfp * kNMfp);
+  A_(c->d_zcur, sizeof(float) * kNXMfp);
+  A_(c->d_hin, sizeof(float) * kNHistory);
+  A_(c->d_chist, sizeof(float) * kNCHistory);
+  A_(c->d_zvz, sizeof(float) * kNHistory);
   A_(c->d_grfA, sizeof(double));
 #undef A_
   if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("device allocation failed: ") + hipGetErrorString(e));
@@ -741,6 +756,171 @@ int prc_d2z_history(prc_d2z_ctx* c, uint32_t* counts) {
   PRC_CHECK(c, hipMemcpyAsync(counts, c->d_hist, sizeof(unsigned) * kNHistory, hipMemcpyDeviceToHost, c->stream),
             "history");
   PRC_CHECK(c, hipStreamSynchronize(c->stream), "history");
+  return PRC_D2Z_OK;
+}
+
+// ----------------------------------------------------- history correction
+namespace {
+
+constexpr double kHistInitial = 49.9, kHistFinal = 0.01;  // HISTORY_INITIAL/FINAL, allvars_d2z.h:2-3
+constexpr double kXMfpInitial = 0.01, kXMfpFinal = 0.999;  // X_MFP_INITIAL/FINAL, allvars_d2z.h:10-11
+
+float history_dz() { return (float)((kHistFinal - kHistInitial) / (kNHistory - 1)); }
+float chistory_dz() { return (float)((kHistFinal - kHistInitial) / (kNCHistory - 1)); }
+float mfp_dx() { return (float)((kXMfpFinal - kXMfpInitial) / (kNXMfp - 1)); }
+
+// Redshift at which the history first crosses x (bubblehistory_d2z.C:113-125):
+// strict bracket history[i] < x < history[i+1], linear in between; below the
+// first value 49.9, above the last 0.01; no bracket keeps the previous value
+// For a non-decreasing history (a float running sum of counts always is) the
+// only bracket is at the last i with history[i] < x, found by bisection.
+float history_redshift(const float* history, float x, float prev, float dzh, bool sorted) {
+  if (x < history[0]) return (float)kHistInitial;
+  if (x > history[kNHistory - 1]) return (float)kHistFinal;
+  int i = -1;
+  if (sorted) {
+    const int q = (int)(std::lower_bound(history, history + kNHistory, x) - history) - 1;
+    if (q >= 0 && q < kNHistory - 1 && history[q + 1] > x) i = q;
+  } else {
+    for (int q = 0; q < kNHistory - 1; ++q)
+      if (x > history[q] && x < history[q + 1]) {
+        i = q;
+        break;
+      }
+  }
+  if (i < 0) return prev;
+  const float f = (x - history[i]) / (history[i + 1] - history[i]);
+  const double z0 = kHistInitial + (double)((float)i * dzh);
+  const double z1 = kHistInitial + (double)((float)(i + 1) * dzh);
+  return (float)((double)(1.0f - f) * z0 + (double)f * z1);
+}
+
+// Float2FloatTable::Float2Float (tables.C:27-38): table[bin] at min + bin delta,
+// delta = (max - min)/n; the last interval holds table[n-1] (the reference
+// reads table[n] there)
+float table_value(const float* t, int n, float tmin, float tmax, float v) {
+  if (v <= tmin) return t[0];
+  if (v >= tmax) return t[n - 1];
+  const float delta = (tmax - tmin) / (float)n;
+  const int bin = (int)((v - tmin) / delta);
+  if (bin >= n - 1) return t[n - 1];
+  const float f = (v - (tmin + (float)bin * delta)) / delta;
+  return (1.0f - f) * t[bin] + f * t[bin + 1];
+}
+
+// BubbleMFP (bubblehistory_d2z.C:246-266): mfp at ionized fraction x, linear
+float bubble_mfp_at(const float* mfp, float x, float dx) {
+  const int itab = (int)(((double)x - kXMfpInitial) / (double)dx);
+  if (itab < 0) return mfp[0];
+  if (itab >= kNXMfp - 1) return mfp[kNXMfp - 1];
+  const float f = (float)(((double)(x - (float)itab * dx) - kXMfpInitial) / (double)dx);
+  return (1.0f - f) * mfp[itab] + f * mfp[itab + 1];
+}
+
+}  // namespace
+
+int prc_d2z_bubble_mfp(prc_d2z_ctx* c, const float* history, int nskewers, uint64_t seed, float* mfp,
+                       float* mfp_x, float* dfdmfp) {
+  if (!c || !history || !mfp) return PRC_D2Z_ERR_BAD_ARG;
+  if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
+  if (nskewers == 0) nskewers = kNSkewers;
+  if (nskewers < 0 || nskewers > (1 << 24)) return c->fail(PRC_D2Z_ERR_BAD_ARG, "nskewers must be 0..2^24");
+  if (c->N < 3) return c->fail(PRC_D2Z_ERR_UNSUPPORTED_N, "skewers need N >= 3");
+  (void)hipSetDevice(c->p.device);
+  // zcur per ionized fraction x = 0.01 + ix dxmfp (bubblehistory_d2z.C:104-125)
+  const float dzh = history_dz(), dx = mfp_dx();
+  std::vector<float> zcur(kNXMfp);
+  float prev = (float)kHistInitial;
+  const bool sorted = std::is_sorted(history, history + kNHistory);
+  for (int ix = 0; ix < kNXMfp; ++ix) {
+    const float x = (float)(kXMfpInitial + (double)((float)ix * dx));
+    prev = zcur[ix] = history_redshift(history, x, prev, dzh, sorted);
+  }
+  // CellSize = BoxSize/N; bins of ln distance between ln CellSize and ln BoxSize
+  // (bubblehistory_d2z.C:95-101)
+  const float cell = c->p.BoxSize / (float)c->N;
+  SkewerParams sp;
+  sp.N = c->N;
+  sp.nskewers = nskewers;
+  sp.seed_lo = (unsigned)(seed & 0xffffffffu);
+  sp.seed_hi = (unsigned)(seed >> 32);
+  sp.cell = (double)cell;
+  sp.lmin = logf(cell);
+  sp.lmax = (float)log(1.0 * (double)c->p.BoxSize);
+  sp.dmfp = (sp.lmax - sp.lmin) / (float)kNMfp;
+  hipStream_t s = c->stream;
+  PRC_CHECK(c, hipMemcpyAsync(c->d_zcur, zcur.data(), sizeof(float) * kNXMfp, hipMemcpyHostToDevice, s), "skewers");
+  PRC_CHECK(c, hipMemsetAsync(c->d_mfp_sum, 0, sizeof(double) * kNXMfp, s), "skewers");
+  PRC_CHECK(c, hipMemsetAsync(c->d_mfp_cnt, 0, sizeof(unsigned) * kNXMfp * kNMfp, s), "skewers");
+  PRC_CHECK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), s), "skewers");
+  PRC_CHECK(c, launch_skewers(c->d_zre, c->d_zcur, sp, c->d_mfp_sum, c->d_mfp_cnt, c->d_flag, s), "skewers");
+  std::vector<double> sums(kNXMfp);
+  std::vector<unsigned> cnt((size_t)kNXMfp * kNMfp);
+  int failed = 0;
+  PRC_CHECK(c, hipMemcpyAsync(sums.data(), c->d_mfp_sum, sizeof(double) * kNXMfp, hipMemcpyDeviceToHost, s),
+            "skewers");
+  PRC_CHECK(c, hipMemcpyAsync(cnt.data(), c->d_mfp_cnt, sizeof(unsigned) * cnt.size(), hipMemcpyDeviceToHost, s),
+            "skewers");
+  PRC_CHECK(c, hipMemcpyAsync(&failed, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s), "skewers");
+  PRC_CHECK(c, hipStreamSynchronize(s), "skewers");
+  if (failed)
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "skewers: no start cell with zreion > zcur in " +
+                                            std::to_string(kMaxStartDraws) + " draws (history does not match the field)");
+  // mean over 2 nskewers walks (bubblehistory_d2z.C:236-240)
+  for (int ix = 0; ix < kNXMfp; ++ix) {
+    mfp[ix] = (float)(sums[ix] / (2.0 * nskewers));
+    if (mfp_x) mfp_x[ix] = (float)(kXMfpInitial + (double)((float)ix * dx));
+  }
+  if (dfdmfp)
+    for (size_t q = 0; q < cnt.size(); ++q) dfdmfp[q] = (float)cnt[q];
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_corrected_history(prc_d2z_ctx* c, const float* fcoll, int nfcoll, float fmin, float fmax,
+                              const float* mfp, float* chistory) {
+  if (!c || !fcoll || !mfp || !chistory) return PRC_D2Z_ERR_BAD_ARG;
+  if (nfcoll < 2 || !(fmax > fmin)) return c->fail(PRC_D2Z_ERR_BAD_ARG, "fcoll table: need n >= 2 and max > min");
+  // GetCHistory (bubblehistory_d2z.C:44-80), float arithmetic as the reference
+  const float dzc = chistory_dz(), dx = mfp_dx();
+  const float rmfp = c->p.Rmax, zeta = c->p.zeta;
+  chistory[0] = 0.0f;
+  for (int i = 1; i < kNCHistory; ++i) {
+    const float x1 = chistory[i - 1];
+    const float z1 = (float)(kHistInitial + (double)((float)i * dzc));
+    const float z2 = (float)(kHistInitial + (double)((float)(i + 1) * dzc));
+    const float f1 = table_value(fcoll, nfcoll, fmin, fmax, z1);
+    const float f2 = table_value(fcoll, nfcoll, fmin, fmax, z2);
+    const float df = f2 - f1;
+    const float rb = bubble_mfp_at(mfp, x1, dx);
+    float v = chistory[i - 1] + df / (1.0f + rb / rmfp) * zeta;
+    if (v >= 1.0f) v = 1.0f;
+    chistory[i] = v;
+  }
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_correct_zreion(prc_d2z_ctx* c, const float* history, const float* chistory) {
+  if (!c || !history || !chistory) return PRC_D2Z_ERR_BAD_ARG;
+  if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
+  (void)hipSetDevice(c->p.device);
+  hipStream_t s = c->stream;
+  std::vector<float> zvz(kNHistory);
+  PRC_CHECK(c, hipMemcpyAsync(c->d_hin, history, sizeof(float) * kNHistory, hipMemcpyHostToDevice, s), "correct");
+  PRC_CHECK(c, hipMemcpyAsync(c->d_chist, chistory, sizeof(float) * kNCHistory, hipMemcpyHostToDevice, s),
+            "correct");
+  PRC_CHECK(c, launch_zvz(c->d_hin, c->d_chist, chistory_dz(), c->d_zvz, s), "correct");
+  PRC_CHECK(c, hipMemcpyAsync(zvz.data(), c->d_zvz, sizeof(float) * kNHistory, hipMemcpyDeviceToHost, s), "correct");
+  PRC_CHECK(c, hipStreamSynchronize(s), "correct");
+  // bins without a value take the next bin's (CorrectHistory, :316)
+  for (int i = 0; i < kNHistory; ++i)
+    if (std::isnan(zvz[i]))
+      return c->fail(PRC_D2Z_ERR_BAD_ARG, "corrected history brackets no value of the history (bin " +
+                                              std::to_string(i) + ")");
+  for (int i = kNHistory - 2; i >= 0; --i)
+    if (zvz[i] < 0.0f) zvz[i] = zvz[i + 1];
+  PRC_CHECK(c, hipMemcpyAsync(c->d_zvz, zvz.data(), sizeof(float) * kNHistory, hipMemcpyHostToDevice, s), "correct");
+  PRC_CHECK(c, launch_remap(c->d_zre, c->ncell, c->d_zvz, (double)history_dz(), s), "correct");
+  PRC_CHECK(c, hipStreamSynchronize(s), "correct");
   return PRC_D2Z_OK;
 }
 

@@ -11,16 +11,19 @@
 //
 //   prc_d2z <paramfile> [-f name] [-r Rmax] [-m Mmin] [-z zeta] [-l] [-v] [-e]
 //                       [-M plain|fused|mixed] [-b radii_per_launch] [-d device]
+//                       [-T fcoll table]
 //
 // Differences from the reference, on purpose:
 //   * every failure prints a message and exits with status 1 (the reference
 //     prints and continues, or exits with status 0);
 //   * -l (lean mode) is accepted and ignored: the device holds the spectrum and
 //     the intermediates, so the extra forward transforms buy nothing;
-//   * the .zreion written is the field of delta2zreion() itself; the history
-//     correction chain of main_d2z.C:79-90 (GetBubbleMFP, GetCHistory,
-//     CorrectHistory) needs fcoll_table.tab from the colossus-based table
-//     builder and is not part of this program.
+//   * the history correction of main_d2z.C:78-90 (GetBubbleMFP, WriteMFP,
+//     WriteMFPPDF, GetCHistory, CorrectHistory) runs as the reference's does and
+//     the .zreion written is the corrected field; the skewers draw Philox
+//     numbers (seed 1) instead of rand().  fcoll_table.tab (or -T <file>) comes
+//     from the colossus-based table builder; without it the program says so and
+//     writes the uncorrected field (the reference crashes on the missing file).
 #include <getopt.h>
 #include <math.h>
 #include <stdio.h>
@@ -44,6 +47,7 @@ struct Options {
   float Rmax = 256, zeta = 10, M_min = 1e8;  // commandline_d2z.C:47-49
   int verbose = 0, everbose = 0, lean = 0;
   int mode = PRC_D2Z_FUSED_FP32, radii_per_launch = 0, device = 0;
+  std::string fcoll_table = "fcoll_table.tab";  // Float2FloatTable file of GetCHistory
 };
 
 struct Param {
@@ -75,6 +79,7 @@ void usage() {
   printf("\n   -M plain|fused|mixed  GPU execution path [default = fused]");
   printf("\n   -b <n> radii per fused batch, 1..32 [default = 32]");
   printf("\n   -d <n> HIP device [default = 0]");
+  printf("\n   -T <file> collapsed-fraction table of the history correction [default = fcoll_table.tab]");
   printf("\n\n");
   exit(0);
 }
@@ -86,7 +91,7 @@ Options command_line(int argc, char** argv) {
   o.paramfile = argv[1];
   opterr = 0;
   int c;
-  while ((c = getopt(argc - 1, &argv[1], "hvelf:r:m:z:M:b:d:")) != -1) {
+  while ((c = getopt(argc - 1, &argv[1], "hvelf:r:m:z:M:b:d:T:")) != -1) {
     switch (c) {
       case 'h': usage();
       case 'f': o.basename = std::string("output/") + optarg; break;
@@ -104,6 +109,7 @@ Options command_line(int argc, char** argv) {
         break;
       case 'b': o.radii_per_launch = atoi(optarg); break;
       case 'd': o.device = atoi(optarg); break;
+      case 'T': o.fcoll_table = optarg; break;
       case '?':
         fprintf(stderr, "\n Unknown option or missing argument `-%c'.\n", optopt);
         usage();
@@ -289,6 +295,50 @@ int main(int argc, char** argv) {
   history_z[0] = (float)kHistoryInitial;
   for (int i = 1; i < kNHistory; ++i) history_z[i] = (float)(kHistoryInitial + i * dzhistory);
   const float tau = tau_es(history_z, history, P.Omegam, P.Omegab, P.h);
+
+  // GetBubbleMFP (bubblehistory_d2z.C:82-242), WriteMFP and WriteMFPPDF
+  // (io_d2z.C:343-400)
+  std::vector<float> mfp(PRC_D2Z_NX_MFP), mfp_x(PRC_D2Z_NX_MFP), dfdmfp((size_t)PRC_D2Z_NX_MFP * PRC_D2Z_NMFP);
+  check(prc_d2z_bubble_mfp(ctx, history.data(), 0, 1, mfp.data(), mfp_x.data(), dfdmfp.data()), ctx, "bubble mfp");
+  printf("\n Writing bubble mfp...");
+  {
+    FILE* f = fopen((o.basename + ".mfp").c_str(), "w");
+    if (!f) die("cannot write %s.mfp", o.basename.c_str());
+    for (int i = 0; i < PRC_D2Z_NX_MFP; ++i) fprintf(f, "%f %f\n", mfp_x[i], mfp[i]);
+    fclose(f);
+  }
+  printf("\n Writing bubble mfp...");
+  {
+    FILE* f = fopen((o.basename + ".mfppdf").c_str(), "wb");
+    if (!f) die("cannot write %s.mfppdf", o.basename.c_str());
+    const float cell = P.BoxSize / (float)P.N;
+    const int header1[2] = {PRC_D2Z_NX_MFP, PRC_D2Z_NMFP};
+    const float header2[4] = {0.01f, 0.999f, log10f(cell), (float)log10(1.0 * (double)P.BoxSize)};
+    if (fwrite(header1, sizeof(int), 2, f) != 2 || fwrite(header2, sizeof(float), 4, f) != 4 ||
+        fwrite(dfdmfp.data(), sizeof(float), dfdmfp.size(), f) != dfdmfp.size())
+      die("cannot write %s.mfppdf", o.basename.c_str());
+    fclose(f);
+  }
+
+  // GetCHistory and CorrectHistory (bubblehistory_d2z.C:44-80,301-343) with the
+  // Float2FloatTable file (tables.C:9-25: float min, float max, int n, float[n])
+  if (FILE* f = fopen(o.fcoll_table.c_str(), "rb")) {
+    float tmin = 0, tmax = 0;
+    int n = 0;
+    if (fread(&tmin, 4, 1, f) != 1 || fread(&tmax, 4, 1, f) != 1 || fread(&n, 4, 1, f) != 1 || n < 2 ||
+        n > (1 << 28))
+      die("malformed table %s", o.fcoll_table.c_str());
+    std::vector<float> table(n);
+    if (fread(table.data(), 4, (size_t)n, f) != (size_t)n) die("short table %s", o.fcoll_table.c_str());
+    fclose(f);
+    std::vector<float> chistory(PRC_D2Z_NCHISTORY);
+    check(prc_d2z_corrected_history(ctx, table.data(), n, tmin, tmax, mfp.data(), chistory.data()), ctx,
+          "corrected history");
+    check(prc_d2z_correct_zreion(ctx, history.data(), chistory.data()), ctx, "history correction");
+  } else {
+    printf("\n %s not found: the history correction is skipped and the uncorrected zreion is written",
+           o.fcoll_table.c_str());
+  }
 
   // WriteZreionFile (io_d2z.C:209-246)
   printf("\n Writing reionization redshifts...");

@@ -30,6 +30,9 @@ LIB_PATH = Path(__file__).resolve().parents[1] / "lib" / "libprc_d2z.so"
 PLAIN_FP32, FUSED_FP32, FUSED_MIXED = 0, 1, 2
 DENSE, PADDED = 0, 1
 NHISTORY = 100000
+NCHISTORY = 10000
+NX_MFP = 1000
+NMFP = 200
 MAX_KERNELS = 16
 
 STATUS = {
@@ -72,6 +75,7 @@ EXPORTS = [
     "prc_d2z_smoothed_field", "prc_d2z_displacement", "prc_d2z_power_spectrum", "prc_d2z_set_timing", "prc_d2z_get_timing", "prc_d2z_device_bytes",
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
+    "prc_d2z_bubble_mfp", "prc_d2z_corrected_history", "prc_d2z_correct_zreion",
 ]
 
 _lib = None
@@ -122,6 +126,9 @@ def load_library(path: str | os.PathLike | None = None):
                                                 C.POINTER(C.c_int)]),
         "prc_d2z_autotune": (C.c_int, [vp, C.POINTER(CTuning)]),
         "prc_d2z_run_many": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "prc_d2z_bubble_mfp": (C.c_int, [vp, fp, C.c_int, C.c_uint64, fp, fp, fp]),
+        "prc_d2z_corrected_history": (C.c_int, [vp, fp, C.c_int, C.c_float, C.c_float, fp, fp]),
+        "prc_d2z_correct_zreion": (C.c_int, [vp, fp, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -243,6 +250,36 @@ class D2Z:
         out = np.zeros(NHISTORY, dtype=np.uint32)
         self._check(self.lib.prc_d2z_history(self._ctx, out))
         return out
+
+    @staticmethod
+    def cumulative_history(counts: np.ndarray, N: int) -> np.ndarray:
+        """GetHistory's float history (bubblehistory_d2z.C:31-32): counts / N^3
+        rounded to float, then the running float sum."""
+        h = (np.asarray(counts, dtype=np.float64) / float(N) ** 3).astype(np.float32)
+        return np.cumsum(h, dtype=np.float32)
+
+    def bubble_mfp(self, history: np.ndarray, nskewers: int = 0, seed: int = 1):
+        """GetBubbleMFP (bubblehistory_d2z.C:82-242) on the GPU: mfp[NX_MFP],
+        mfp_x[NX_MFP], dfdmfp[NX_MFP, NMFP]."""
+        mfp = np.empty(NX_MFP, dtype=np.float32)
+        mfp_x = np.empty(NX_MFP, dtype=np.float32)
+        dfdmfp = np.empty((NX_MFP, NMFP), dtype=np.float32)
+        self._check(self.lib.prc_d2z_bubble_mfp(self._ctx, np.ascontiguousarray(history, dtype=np.float32),
+                                                nskewers, seed, mfp, mfp_x, dfdmfp))
+        return mfp, mfp_x, dfdmfp
+
+    def corrected_history(self, fcoll: np.ndarray, fmin: float, fmax: float, mfp: np.ndarray) -> np.ndarray:
+        """GetCHistory (bubblehistory_d2z.C:44-80): chistory[NCHISTORY]."""
+        t = np.ascontiguousarray(fcoll, dtype=np.float32)
+        out = np.empty(NCHISTORY, dtype=np.float32)
+        self._check(self.lib.prc_d2z_corrected_history(self._ctx, t, len(t), fmin, fmax,
+                                                       np.ascontiguousarray(mfp, dtype=np.float32), out))
+        return out
+
+    def correct_zreion(self, history: np.ndarray, chistory: np.ndarray):
+        """CorrectHistory (bubblehistory_d2z.C:301-343) on the context's zreion."""
+        self._check(self.lib.prc_d2z_correct_zreion(self._ctx, np.ascontiguousarray(history, dtype=np.float32),
+                                                    np.ascontiguousarray(chistory, dtype=np.float32)))
 
     def smoothed_field(self, r: int) -> np.ndarray:
         N = self.N

@@ -66,3 +66,50 @@ def test_driver_rejects_bad_parameter_file(tmp_path):
     r = subprocess.run([str(EXE), "p.d2z", "-v", "-e"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "Tag 'Bogus' not allowed or multiple defined." in r.stdout
+
+
+def test_driver_history_correction(tmp_path):
+    """main_d2z.C:78-90: .mfp and .mfppdf written, and the .zreion corrected
+    through fcoll_table.tab; without the table the uncorrected field."""
+    import bubble as ob
+
+    g = np.load(FIX, allow_pickle=False)
+    N, box = int(g["N"]), float(g["BoxSize"])
+    g["delta"].astype(np.float32).tofile(tmp_path / "delta")
+    (tmp_path / "param.d2z").write_text(PARAM.format(box=box, N=N, delta=tmp_path / "delta",
+                                                     ns=int(g["Nscales"])))
+    n, tmin, tmax = 1000, 0.0, 50.0
+    zt = tmin + (tmax - tmin) / n * np.arange(n)
+    table = (0.03 * np.exp(-((zt / 9.0) ** 2))).astype(np.float32)
+    with open(tmp_path / "fcoll_table.tab", "wb") as f:
+        np.array([tmin, tmax], np.float32).tofile(f)
+        np.array([n], np.int32).tofile(f)
+        table.tofile(f)
+    announce("driver history correction")
+    base = [str(EXE), "param.d2z", "-m", str(float(g["M_min"])), "-z", str(float(g["zeta"])),
+            "-r", str(float(g["Rmax"])), "-v", "-e", "-M", "fused"]
+    r0 = subprocess.run(base + ["-f", "raw", "-T", "missing.tab"], cwd=tmp_path, capture_output=True, text=True,
+                        timeout=120)
+    assert r0.returncode == 0 and "missing.tab not found" in r0.stdout
+    r1 = subprocess.run(base + ["-f", "cor"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r1.stdout[-1500:], r1.stderr[-1500:])
+    assert r1.returncode == 0
+    raw = np.fromfile(tmp_path / "output" / "raw.zreion", dtype=np.float32).reshape(N, N, N)
+    cor = np.fromfile(tmp_path / "output" / "cor.zreion", dtype=np.float32).reshape(N, N, N)
+    hist = np.cumsum((oracle.history_counts(raw) / float(N) ** 3).astype(np.float32), dtype=np.float32)
+    mfp, mfp_x, dfd, _, _ = ob.bubble_mfp(raw, hist, box, nskewers=ob.NSKEWERS, seed=1)
+    txt = np.loadtxt(tmp_path / "output" / "cor.mfp")
+    assert txt.shape == (ob.NX_MFP, 2)
+    np.testing.assert_allclose(txt[:, 0], mfp_x, atol=1e-6)
+    np.testing.assert_allclose(txt[:, 1], mfp, rtol=1e-5, atol=2e-6)
+    with open(tmp_path / "output" / "cor.mfppdf", "rb") as f:
+        h1 = np.fromfile(f, np.int32, 2)
+        h2 = np.fromfile(f, np.float32, 4)
+        pdf = np.fromfile(f, np.float32)
+    assert list(h1) == [ob.NX_MFP, ob.NMFP]
+    np.testing.assert_allclose(h2, [0.01, 0.999, np.log10(box / N), np.log10(box)], rtol=1e-6)
+    assert pdf.size == ob.NX_MFP * ob.NMFP and np.abs(pdf - dfd.ravel()).sum() <= 4
+    ch = ob.corrected_history(table, tmin, tmax, mfp, float(g["Rmax"]), float(g["zeta"]))
+    ref = ob.correct_zreion(raw, hist, ch)
+    assert np.max(np.abs(cor - ref)) < 1e-3
+    assert not np.array_equal(cor, raw)
