@@ -105,6 +105,33 @@ def cpu_baseline_sample(delta: np.ndarray, lp, nthreads: int) -> dict:
                       f"rate extrapolated to {NS} radii"}
 
 
+def history_correction_times(d, N):
+    """Wall times (ms) of the history correction on the measured field: the
+    calls of main_d2z.C:78-90 in order.  fcoll(z) is a synthetic table
+    (0.03 exp(-(z/9)^2) on [0, 50], 2000 entries): the real one comes from the
+    reference's colossus-based script."""
+    times = {}
+
+    def timed(name, f):
+        t0 = time.perf_counter()
+        r = f()
+        times[name] = round((time.perf_counter() - t0) * 1e3, 3)
+        return r
+
+    from prc_d2z import D2Z
+
+    counts = timed("history_ms", d.history)
+    hist = D2Z.cumulative_history(counts, N)
+    mfp, _, dfd = timed("bubble_mfp_ms", lambda: d.bubble_mfp(hist))
+    zt = np.arange(2000) * (50.0 / 2000)
+    table = (0.03 * np.exp(-((zt / 9.0) ** 2))).astype(np.float32)
+    ch = timed("corrected_history_ms", lambda: d.corrected_history(table, 0.0, 50.0, mfp))
+    timed("correct_zreion_ms", lambda: d.correct_zreion(hist, ch))
+    times["skewer_walks"] = 2 * 20000 * 1000
+    times["binned_walks"] = int(dfd.sum())
+    return times
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +148,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--history", action="store_true",
                     help="also run the GetHistory histogram once after timing (read-only pass over zreion)")
+    ap.add_argument("--history-correction", action="store_true",
+                    help="after timing, run the history correction once (GetHistory, bubble mean free "
+                         "paths, corrected history from a synthetic fcoll(z) table, remap of zreion) and "
+                         "report its wall times under history_correction")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -192,6 +223,9 @@ def main():
     if args.history:
         h = d.history()
         print(f"history: {int(h.sum())} cells counted", file=sys.stderr)
+    hc = None
+    if args.history_correction:
+        hc = history_correction_times(d, N)
     plain = None
     if args.mode != "plain" and not args.no_plain_compare:
         plain = plain_fp32_rate(lp, steps=min(args.steps, 2))
@@ -220,6 +254,7 @@ def main():
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (value / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
+        **({"history_correction": hc} if hc else {}),
     }
     print(json.dumps(out))
 
