@@ -142,7 +142,8 @@ hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long to
 hipError_t launch_moments(const float* x, long long total, double2* partial, int nblk, hipStream_t s);
 hipError_t launch_moments_finish(const double2* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
                                  int r0, hipStream_t s);
-hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, hipStream_t s);
+hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, int* range,
+                          hipStream_t s);
 // history correction (bubble_kernels.hip)
 hipError_t launch_skewers(const float* zre, const float* zcur, const SkewerParams& p, double* sums,
                           unsigned int* counts, int* fail, hipStream_t s);

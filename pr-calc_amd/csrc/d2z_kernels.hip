@@ -96,23 +96,80 @@ __global__ void k_nonfinite(const float* __restrict__ x, long long n, int* __res
 
 // ------------------------------------------------ history histogram (GPU)
 // bin = (int)((z - 49.9)/dzhistory) in double (bubblehistory_d2z.C:17-28);
-// integer counts.  Block (chunk, window) counts the bins of its window in an
-// LDS histogram (LDS atomics), then adds the non-zero bins to the global counts.
-constexpr int kHistWindow = 25000;  // bins per window: 4 windows cover NHISTORY
+// integer counts.  A first pass finds the lowest and highest occupied bin; then
+// block (chunk, w) counts the bins of window [lo + w W, lo + (w + 1) W) in an LDS
+// histogram (LDS atomics) and adds its non-zero bins to the global counts.
+// Windows past the highest bin return at once, so a field whose bins span
+// fewer than W (the usual case: zreion between ~5 and ~20 spans ~30000) is read
+// twice in all.
+constexpr int kHistWindow = 36000;  // bins per window: 144 KB of LDS
+constexpr int kHistWindows = (kNHistory + kHistWindow - 1) / kHistWindow;
+
+__device__ __forceinline__ int history_bin(float z, double dz) {
+  const double q = ((double)z - 49.9) / dz;
+  return (q > -1.0 && q < (double)kNHistory) ? (int)q : -1;  // NaN: -1
+}
+
+__global__ void __launch_bounds__(256) k_history_range(const float* __restrict__ zre, long long total,
+                                                       int* __restrict__ range, double dz) {
+  int lo = kNHistory, hi = -1;
+  const long long n4 = total >> 2;
+  const float4* z4 = reinterpret_cast<const float4*>(zre);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n4; id += stride) {
+    const float4 v = z4[id];
+    const int b[4] = {history_bin(v.x, dz), history_bin(v.y, dz), history_bin(v.z, dz), history_bin(v.w, dz)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (b[q] >= 0) {
+        lo = min(lo, b[q]);
+        hi = max(hi, b[q]);
+      }
+  }
+  for (long long id = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total; id += stride) {
+    const int b = history_bin(zre[id], dz);
+    if (b >= 0) {
+      lo = min(lo, b);
+      hi = max(hi, b);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && hi >= 0) {
+    atomicMin(&range[0], lo);
+    atomicMax(&range[1], hi);
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_history(const float* __restrict__ zre, long long total,
-                                                  unsigned int* __restrict__ counts, double dz) {
+                                                  unsigned int* __restrict__ counts, double dz,
+                                                  const int* __restrict__ range) {
   __shared__ unsigned int h[kHistWindow];
-  const int w = blockIdx.y;
-  const int lo = w * kHistWindow;
+  if (range[1] < 0) return;  // no cell in any bin
+  const int lo = range[0] + (int)blockIdx.y * kHistWindow;
+  if (lo > range[1]) return;  // window past the highest bin (block-uniform)
   for (int b = threadIdx.x; b < kHistWindow; b += blockDim.x) h[b] = 0u;
   __syncthreads();
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total;
-       id += (long long)gridDim.x * blockDim.x) {
-    const double q = ((double)zre[id] - 49.9) / dz;
-    if (q > -1.0 && q < (double)kNHistory) {
-      const int b = (int)q - lo;
-      if (b >= 0 && b < kHistWindow) atomicAdd(&h[b], 1u);
+  const long long n4 = total >> 2;
+  const float4* z4 = reinterpret_cast<const float4*>(zre);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n4; id += 2 * stride) {
+    const float4 v = z4[id];
+    const float4 u = id + stride < n4 ? z4[id + stride] : make_float4(NAN, NAN, NAN, NAN);
+    const int b[8] = {history_bin(v.x, dz), history_bin(v.y, dz), history_bin(v.z, dz), history_bin(v.w, dz),
+                      history_bin(u.x, dz), history_bin(u.y, dz), history_bin(u.z, dz), history_bin(u.w, dz)};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int r = b[q] - lo;
+      if (b[q] >= 0 && r >= 0 && r < kHistWindow) atomicAdd(&h[r], 1u);
     }
+  }
+  for (long long id = 4 * n4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; id < total; id += stride) {
+    const int b = history_bin(zre[id], dz), r = b - lo;
+    if (b >= 0 && r >= 0 && r < kHistWindow) atomicAdd(&h[r], 1u);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < kHistWindow; b += blockDim.x)
@@ -411,11 +468,21 @@ hipError_t launch_moments_finish(const double2* partial, int nblk, int nrad, dou
   return hipGetLastError();
 }
 
-hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, hipStream_t s) {
-  static_assert(4 * kHistWindow == kNHistory, "windows must tile the history bins");
-  long long g = (total + 1023) / 1024;
-  if (g > 256) g = 256;
-  hipLaunchKernelGGL(k_history, dim3((unsigned)g, 4), dim3(1024), 0, s, zre, total, counts, dz);
+hipError_t launch_history(const float* zre, long long total, unsigned int* counts, double dz, int* range,
+                          hipStream_t s) {
+  // range = {0x7f7f7f7f (above every bin), -1} before the min/max pass
+  hipError_t e = hipMemsetAsync(range, 0x7f, sizeof(int), s);
+  if (e == hipSuccess) e = hipMemsetAsync(range + 1, 0xff, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  long long g = (total / 4 + 255) / 256;
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_history_range, dim3((unsigned)g), dim3(256), 0, s, zre, total, range, dz);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  long long gh = (total / 8 + 1023) / 1024;
+  if (gh > 256) gh = 256;
+  if (gh < 1) gh = 1;
+  hipLaunchKernelGGL(k_history, dim3((unsigned)gh, kHistWindows), dim3(1024), 0, s, zre, total, counts, dz, range);
   return hipGetLastError();
 }
 

@@ -114,6 +114,7 @@ struct prc_d2z_ctx {
   double* d_Rint = nullptr;
   float* d_R = nullptr;
   unsigned* d_hist = nullptr;
+  int* d_hrange = nullptr;  // lowest and highest occupied history bin
   double* d_pk = nullptr;  // power spectrum bins: counts (uint64) then 3 sums, kPkMaxBins each
   int* d_flag = nullptr;
   // history correction: skewer sums and counts per ionized fraction, zcur per
@@ -557,6 +558,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_Rint, sizeof(double) * (NS + 2));
   A_(c->d_R, sizeof(float) * NS);
   A_(c->d_hist, sizeof(unsigned) * kNHistory);
+  A_(c->d_hrange, sizeof(int) * 2);
   A_(c->d_pk, sizeof(double) * 4 * kPkMaxBins);
   A_(c->d_flag, sizeof(int));
   A_(c->d_mfp_sum, sizeof(double) * kNXMfp);
@@ -752,7 +754,7 @@ int prc_d2z_history(prc_d2z_ctx* c, uint32_t* counts) {
   // dzhistory = (HISTORY_FINAL - HISTORY_INITIAL)/(NHISTORY-1) stored as float
   const float dzf = (float)((0.01 - 49.9) / (kNHistory - 1));
   PRC_CHECK(c, hipMemsetAsync(c->d_hist, 0, sizeof(unsigned) * kNHistory, c->stream), "history");
-  PRC_CHECK(c, launch_history(c->d_zre, c->ncell, c->d_hist, (double)dzf, c->stream), "history");
+  PRC_CHECK(c, launch_history(c->d_zre, c->ncell, c->d_hist, (double)dzf, c->d_hrange, c->stream), "history");
   PRC_CHECK(c, hipMemcpyAsync(counts, c->d_hist, sizeof(unsigned) * kNHistory, hipMemcpyDeviceToHost, c->stream),
             "history");
   PRC_CHECK(c, hipStreamSynchronize(c->stream), "history");

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import bubble as ob
+import d2z_fp64 as oracle
 from _cases import field, lib_params
 from conftest import announce
 from prc_d2z import D2Z, D2ZError, FUSED_FP32, FUSED_MIXED, NCHISTORY, NMFP, NX_MFP
@@ -59,6 +60,11 @@ def test_history_correction_vs_oracle(N, mode, nsk):
         zc = d.zreion()
         zc_o = ob.correct_zreion(z, hist, ch)
         assert np.array_equal(zc, zc_o)
+        # GetHistory of the corrected field, whose bins span more than one LDS
+        # window of the histogram kernel
+        b = oracle.history_bins(zc)
+        assert b.max() - b.min() > 36000
+        assert np.array_equal(d.history().astype(np.int64), oracle.history_counts(zc))
         # the correction maps each history bin to one redshift and keeps the order of cells
         o = np.argsort(z.ravel(), kind="stable")
         assert np.all(np.diff(zc.ravel()[o]) <= 0) or np.all(np.diff(zc.ravel()[o]) >= 0)
