@@ -4,6 +4,7 @@ import ctypes as C
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 import prc_d2z
@@ -67,3 +68,14 @@ def test_null_context_calls_fail_cleanly():
     assert lib.prc_d2z_sync(None) == -1
     lib.prc_d2z_destroy(None)
     assert lib.prc_d2z_device_bytes(None) == 0
+
+
+def test_history_correction_entry_points_reject_null():
+    lib = prc_d2z.load_library()
+    h = np.zeros(prc_d2z.NHISTORY, np.float32)
+    mfp = np.ones(prc_d2z.NX_MFP, np.float32)
+    ch = np.zeros(prc_d2z.NCHISTORY, np.float32)
+    dfd = np.zeros(prc_d2z.NX_MFP * prc_d2z.NMFP, np.float32)
+    assert lib.prc_d2z_bubble_mfp(None, h, 0, 1, mfp, mfp, dfd) == -1
+    assert lib.prc_d2z_corrected_history(None, mfp, len(mfp), 0.0, 50.0, mfp, ch) == -1
+    assert lib.prc_d2z_correct_zreion(None, h, ch) == -1
