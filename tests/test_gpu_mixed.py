@@ -17,7 +17,9 @@ from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_MIXED
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N", [64, 128])
+# 192 runs the radix-3 plans (z c2r of length 96); every size but 1024 runs the
+# VALU z pass with its packed FP16 glue
+@pytest.mark.parametrize("N", [32, 64, 128, 192, 256])
 def test_mixed_zreion_vs_oracle(N):
     lp = lib_params(N, nscales=10, mode=FUSED_MIXED)
     announce(f"mixed zreion N={N}")
