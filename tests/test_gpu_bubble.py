@@ -17,7 +17,7 @@ import bubble as ob
 import d2z_fp64 as oracle
 from _cases import field, lib_params
 from conftest import announce
-from prc_d2z import D2Z, D2ZError, FUSED_FP32, FUSED_MIXED, NCHISTORY, NMFP, NX_MFP
+from prc_d2z import D2Z, D2ZError, D2ZParams, FUSED_FP32, FUSED_MIXED, NCHISTORY, NMFP, NX_MFP
 
 pytestmark = pytest.mark.gpu
 
@@ -97,3 +97,29 @@ def test_history_correction_errors():
         # the context still works after the failures
         mfp, _, _ = d.bubble_mfp(D2Z.cumulative_history(d.history(), N), nskewers=256)
         assert np.all(np.isfinite(mfp)) and np.all(mfp > 0)
+
+
+def test_history_correction_1024():
+    """The chain at 1024^3 (the mixed headline run): the same draws and walk
+    lengths as the oracle on the full field, the corrected field bit-exact."""
+    N, nsk = 1024, 2000
+    lp = D2ZParams(N=N, BoxSize=2000.0, Nscales=96, Rmax=120.0, zeta=40.0, M_min=1.4706e8, mode=FUSED_MIXED)
+    announce(f"history correction N={N} nskewers={nsk}")
+    with D2Z(lp) as d:
+        d.generate_delta(1)
+        d.run()
+        z = d.zreion()
+        hist = D2Z.cumulative_history(d.history(), N)
+        mfp, mfp_x, dfd = d.bubble_mfp(hist, nskewers=nsk, seed=3)
+        mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, lp.BoxSize, nskewers=nsk, seed=3)
+        assert np.array_equal(mfp_x, x_o)
+        assert np.abs(dfd.astype(np.int64) - dfd_o.astype(np.int64)).sum() <= 4
+        np.testing.assert_allclose(mfp, mfp_o, rtol=1e-6)
+        t, fmin, fmax = _fcoll_table()
+        ch = d.corrected_history(t, fmin, fmax, mfp)
+        assert np.array_equal(ch, ob.corrected_history(t, fmin, fmax, mfp, lp.Rmax, lp.zeta))
+        d.correct_zreion(hist, ch)
+        zc = d.zreion()
+    zc_o = ob.correct_zreion(z, hist, ch)
+    assert np.array_equal(zc, zc_o)
+    print(f"N={N}: mfp(x=0.5) {mfp[495]:.3f} Mpc/h, binned walks {int(dfd.sum())}")
