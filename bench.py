@@ -102,7 +102,22 @@ def cpu_baseline_sample(delta: np.ndarray, lp, nthreads: int) -> dict:
     return {"value": rate, "unit": "cell*radius/s", "cores": nthreads, "kind": "port",
             "sample": f"{N}^3 forward r2c + 1 of {NS} radii (r={NS // 2}) with the C FP32 restatement "
                       f"({res['fft']} FFT); t_forward={t_fwd:.2f}s t_radius={t_rad:.2f}s, "
-                      f"rate extrapolated to {NS} radii"}
+                      f"rate extrapolated to {NS} radii",
+            "host": host_cpu()}
+
+
+def host_cpu() -> str:
+    """CPU model and logical CPU count of the host the baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
 
 
 def history_correction_times(d, N):
