@@ -166,6 +166,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// The same in float: the FP16-storage paths sum their per-radius squares over
+// a wave in float (64 lanes; relative error far below their own) and widen to
+// double for the sums over waves and blocks
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 // ------------------------------------------------------ FP16 complex pairs
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(512)
           best[e].y = zmax_nonneg(best[e].y, z1);
         }
       }
-      const double wsum = wave_sum((double)acc * sc2);
+      const double wsum = (double)wave_sum_f(acc) * sc2;
       if (l == 0) red[w][q] += wsum;
     }
     // zreion (and rbubble) of row base + w, element pair n = l + 64 m (+ 256)

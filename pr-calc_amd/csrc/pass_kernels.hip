@@ -493,7 +493,8 @@ struct FxCols {
 // intermediates are transformed in turn, d2z() is evaluated per cell, the
 // maximum over the batch stays in registers (first radius wins ties) and
 // zreion (and rbubble) is read and written once per batch.  Per-radius sums of
-// delta_R^2 (float products, double sums as GetMeanSigma) are kept per wave in
+// delta_R^2 (float products, double sums as GetMeanSigma; with FP16 storage
+// the wave's own sum is float) are kept per wave in
 // LDS and written as one partial per block and radius.
 template <class P, int C, bool RB, class TS>
 __global__ void __launch_bounds__(256)
@@ -599,7 +600,7 @@ __global__ void __launch_bounds__(256)
           best[e].y = zmax_nonneg(best[e].y, z1);
         }
       }
-      const double wsum = wave_sum(valid ? (double)acc * sc2 : 0.0);
+      const double wsum = kH ? (double)wave_sum_f(valid ? acc : 0.0f) * sc2 : wave_sum(valid ? (double)acc * sc2 : 0.0);
       if (lane == 0) red[w][q] += wsum;
     }
     if (valid) {
