@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(512)
   __shared__ uint2 s_pq[kLz];        // glue constants P, Q (FP16 pairs), k < N/2
   __shared__ float2 s_tw2[kLz / 2];  // exp(2 pi i b / (N/2)) (stage C)
   __shared__ double red[kRowsZ][kFusedMaxRadii];
+  __shared__ BarrierF32 s_bk[kFusedMaxRadii];  // barrier constants of the batch's radii
   // operand matrices in the A layout, per lane: stage A [h][lane], stage B
   // [wave][t][h][lane] (k = 2 wave + t); read per radius rather than held in
   // registers across radii
@@ -288,6 +289,7 @@ __global__ void __launch_bounds__(512)
     s_pq[q] = c2r_glue_pq(q, tw_full[q]);
   }
   for (int q = tid; q < kLz / 2; q += 512) s_tw2[q] = tw_full[2 * q];
+  if (tid < nb) s_bk[tid] = barrier_f32(rc[r0 + tid].A, rc[r0 + tid].B, iscale);
   if (w < 2) s_gA[w][l] = mats[mat_set_a(w) * 64 + l];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -392,7 +394,7 @@ __global__ void __launch_bounds__(512)
         v[m + 4] = csub(u0, u1);
       }
       __syncthreads();  // the raw buffer is reloaded next radius
-      const BarrierF32 bk = barrier_f32(rc[r0 + q].A, rc[r0 + q].B, iscale);
+      const BarrierF32 bk = s_bk[q];
       const float R = rc[r0 + q].R;
       float acc = 0.0f;
 #pragma unroll
