@@ -89,6 +89,15 @@ typedef struct {
   float sigma_grid; /* rms of the smoothed grid */
 } prc_d2z_radius_row;
 
+/* The same row with the barrier constants of d2z() (delta2zreion_d2z.C:206-216):
+ * z = A (delta_R + B) - 1, A = (1+zHigh) D(zHigh)/deltac, B = sqrt(2 (sigma_min^2 -
+ * sigma^2)) erfizeta; B is NaN where sigma(R) > sigma_min, as the reference's
+ * square root (such radii never raise zreion). */
+typedef struct {
+  float R_mpch, sigma, sigma_grid;
+  double A, B;
+} prc_d2z_radius_diag;
+
 /* Kernel timing of the last prc_d2z_run, from HIP events recorded on the
  * context's stream around every launch. */
 #define PRC_D2Z_MAX_KERNELS 16
@@ -143,6 +152,13 @@ int prc_d2z_get_zreion(prc_d2z_ctx* ctx, float* out, int layout);
 int prc_d2z_get_rbubble(prc_d2z_ctx* ctx, float* out, int layout);
 int prc_d2z_get_scalars(prc_d2z_ctx* ctx, prc_d2z_scalars* out);
 int prc_d2z_get_radius_table(prc_d2z_ctx* ctx, prc_d2z_radius_row* rows, int n);
+int prc_d2z_get_radius_diag(prc_d2z_ctx* ctx, prc_d2z_radius_diag* rows, int n);
+/* Pins the scalar set-up of delta2zreion (delta2zreion_d2z.C:17-26,97-100) to
+ * host values, e.g. the reference's own GSL results: sigma[0] = sigma_min,
+ * sigma[1 + r] = sigma(R_r) (r < Nscales), *erfizeta = inverse_erfc(1/zeta);
+ * either pointer may be NULL to keep the computed values.  B_R is rebuilt from
+ * them; the next prc_d2z_run uses them. */
+int prc_d2z_set_scalar_overrides(prc_d2z_ctx* ctx, const float* sigma /* [Nscales + 1] */, const float* erfizeta);
 /* Device pointer of the dense zreion field (valid until destroy). */
 int prc_d2z_zreion_device(prc_d2z_ctx* ctx, const float** d_zreion);
 
@@ -162,9 +178,10 @@ int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */)
  * the first cell with zreion < zcur_i, and record a distance log-uniform in
  * [m - 1, m] cells.  Random numbers: Philox4x32-10 keyed by seed, counter
  * (skewer, i, draw) -- the reference's rand() sequence is not reproduced.
- * mfp[i] = mean distance (BoxSize/N units of the parameter file), mfp_x[i] =
+ * mfp[i] = mean distance in Mpc (the cell is BoxSize/h/N: the d2z program
+ * divides the parameter file's BoxSize by h first, main_d2z.C:29), mfp_x[i] =
  * x_i, dfdmfp[i * PRC_D2Z_NMFP + b] = walks in ln-distance bin b between
- * ln(BoxSize/N) and ln BoxSize (mfp_x, dfdmfp nullable).  A skewer that finds
+ * ln(cell) and ln(BoxSize/h) (mfp_x, dfdmfp nullable).  A skewer that finds
  * no start cell in 16384 draws fails the call (the reference loops forever). */
 int prc_d2z_bubble_mfp(prc_d2z_ctx* ctx, const float* history, int nskewers, uint64_t seed, float* mfp,
                        float* mfp_x, float* dfdmfp);

@@ -126,6 +126,37 @@ def growth(z, omegam, omegal, w):
     return f32(f32(g / x) / g0)
 
 
+def deltac(z, omegam, omegal, w):
+    """Linear collapse threshold over D(z), cosmology.C:141-172: the open-universe
+    (Omega_Lambda = 0, Omega_m < 1) or the flat fitting form of the threshold,
+    float variables with the double sub-expressions of the C++ (pow with a double
+    or int exponent, the 1.5/3/20/0.0123 factors)."""
+    z, omegam, omegal = f32(z), f32(omegam), f32(omegal)
+    pi = f32(3.14159265358979)
+    x = f32(f32(1) + z)
+    x2 = f32(x * x)
+    x3 = f32(f32(x * x) * x)
+    curv = f32(f32(f32(1) - omegam) - omegal)
+    omega = f32(f32(omegam * x3) / f32(f32(f32(omegam * x3) + f32(curv * x2)) + omegal))
+    u = f32(f32(f32(2) / omega) - f32(1))
+    eta = f32(math.log(f64(u) + math.sqrt(f64(u) ** 2 - 1)))
+    if omegam < 1 and omegal == 0:
+        sh, ch = f32(np.sinh(eta)), f32(np.cosh(eta))
+        a = f32(f32(f32(3) * sh) * f32(sh - eta))
+        dc = f32(1.5 * (f64(a) / f64(f32(ch - f32(1))) ** 2 - 2)
+                 * (1 + f64(f32(f32(f32(2) * pi) / f32(sh - eta))) ** (2.0 / 3.0)))
+    else:
+        dc = f32(3 * f64(f32(f32(12) * pi)) ** (2.0 / 3.0) / 20 * (1 + 0.0123 * f64(f32(np.log10(omega)))))
+    return f32(dc / growth(z, omegam, omegal, w))
+
+
+def fcoll(z, omegam, omegal, w, sigma_min):
+    """Collapsed fraction erfc(deltac(z) / sqrt(2) / sigma_min), cosmology.C:311-324
+    (the fourth column of WriteHistory, io_d2z.C:284-314)."""
+    arg = f32(f64(deltac(z, omegam, omegal, w)) / math.sqrt(2.0) / f64(f32(sigma_min)))
+    return f32(math.erfc(f64(arg)))
+
+
 class _EH:
     """Common block of cosmology.C:14 (omega0, baryon0, hubble0, an0, pnorm0)."""
 

@@ -120,6 +120,19 @@ __device__ __forceinline__ float filter_of_kmag(float km, float dkR, float invN3
   return tophat(km * dkR) * invN3;
 }
 
+// ------------------------------------------- per-radius barrier offset B_R
+// B = sqrt(2 (smin^2 - sigma^2)) * erfizeta with the float difference widened
+// to double (delta2zreion_d2z.C:211).  Where sigma(R) > sigma_min the
+// reference's square root is NaN, its z is NaN and its strict `zcur >
+// zreion` never updates (delta2zreion_d2z.C:117,138): such a radius
+// contributes nothing.  Here B is -inf instead, so z = -inf, which no form of
+// the running maximum (strict compare, fmaxf, or the integer max of
+// zmax_nonneg, where a positive NaN would win) lets through.
+__host__ __device__ inline double barrier_offset(float dd, float erfizeta) {
+  if (!(dd >= 0.0f)) return -INFINITY;
+  return sqrt(2. * (double)dd) * (double)erfizeta;
+}
+
 // -------------------------------------- barrier: d2z() in double, stored float
 // z = A*(delta_R + B) - 1 with A float and B = sqrt(2 (smin^2 - s^2)) * erfizeta
 // in double (delta2zreion_d2z.C:206-216); separate multiply and subtract (no
@@ -155,8 +168,11 @@ __device__ __forceinline__ float barrier_z_f32(float d, BarrierF32 k) { return f
 // Running maximum of barrier values for zreion, which starts at 0 and is only
 // raised: values <= 0 never change it, so the maximum may start at 0, and for
 // non-negative floats the signed-integer order of the bit patterns is the float
-// order (a negative z has the sign bit set and compares below all of them).  One
-// integer max, where fmaxf of a loop-carried value costs a canonicalizing max too.
+// order (a negative z, -inf included, has the sign bit set and compares below
+// all of them).  z must not be NaN (a positive NaN's bits exceed every finite
+// float's): barrier_offset makes the radii whose reference z is NaN give -inf.
+// One integer max, where fmaxf of a loop-carried value costs a canonicalizing
+// max too.
 __device__ __forceinline__ float zmax_nonneg(float best, float z) {
   return __int_as_float(max(__float_as_int(best), __float_as_int(z)));
 }

@@ -73,8 +73,8 @@ struct WTable {
 };
 constexpr int kWTableMax = 3584;  // intervals (56 KB of LDS)
 // intervals and spacing covering [0, xmax]: h = 1/8 if it fits in
-// max_intervals, else 1/4, else 1/2; 0 intervals if none does.  Max |error|
-// of the interpolant (W(0) = 1): 9.6e-8 at h = 1/8, 9.0e-7 at 1/4, 1.4e-5 at 1/2
+// max_intervals, else 1/4; 0 intervals (exact W) if neither does.  Max |error|
+// of the interpolant (W(0) = 1): 9.6e-8 at h = 1/8, 9.0e-7 at 1/4
 int wtable_intervals(double xmax, int max_intervals, double* h);
 hipError_t launch_wtable(float4* coef, int M, double h, hipStream_t s);
 int block_width(int N);

@@ -413,7 +413,7 @@ __global__ void k_setup_finish(SetupParams p, const double* __restrict__ integ, 
     c.R = Rr[r];
     c.sigma = sigma;
     c.A = (double)s_A;
-    c.B = sqrt(2. * (double)dd) * (double)s_erfz;
+    c.B = barrier_offset(dd, s_erfz);
     rc[r] = c;
     sigma_out[r] = sigma;
   }
@@ -539,7 +539,9 @@ __global__ void k_wtable(float4* __restrict__ coef, int M, double h) {
 }
 
 int wtable_intervals(double xmax, int max_intervals, double* h) {
-  for (double hh : {0.125, 0.25, 0.5}) {
+  // h = 1/2 would err by 1.4e-5 (~200 FP32 ulps of W): beyond h = 1/4 the
+  // kernels evaluate W exactly instead
+  for (double hh : {0.125, 0.25}) {
     const double m = ceil(xmax / hh) + 1.0;
     if (m <= max_intervals) {
       *h = hh;

@@ -595,7 +595,8 @@ __global__ void __launch_bounds__(256)
             best[e].y = z1;
             bR1[e] = R;
           }
-        } else {  // same zreion as the strict comparison (z is never NaN)
+        } else {  // same zreion as the strict comparison: z is never NaN (a radius
+                  // with sigma(R) > sigma_min has B = -inf, see barrier_offset)
           best[e].x = zmax_nonneg(best[e].x, z0);
           best[e].y = zmax_nonneg(best[e].y, z1);
         }

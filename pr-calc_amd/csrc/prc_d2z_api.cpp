@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/prc_d2z.h"
+#include "d2z_device.h"
 #include "d2z_kernels.h"
 
 using namespace prc;
@@ -203,10 +204,13 @@ int default_fz_blocks(const prc_d2z_params& p) {
   return fz_mfma_supported(p.N, half) ? fz_mfma_blocks(rb) : fz_blocks(p.N, half, rb);
 }
 
+// BoxSize in Mpc as the d2z program holds it after Parameters.BoxSize /= h
+// (main_d2z.C:29, float division)
+float box_mpc(const prc_d2z_params& p) { return p.BoxSize / p.h; }
+
 float host_dk(const prc_d2z_ctx* c) {
   // float dk = 2.*M_PI/BoxSize with BoxSize already divided by h (delta2zreion_d2z.C:41)
-  const float box = c->p.BoxSize / c->p.h;
-  return (float)(2. * M_PI / (double)box);
+  return (float)(2. * M_PI / (double)box_mpc(c->p));
 }
 
 // Plain path, one radius: separate filter, three inverse passes, moments and
@@ -747,6 +751,54 @@ int prc_d2z_get_radius_table(prc_d2z_ctx* c, prc_d2z_radius_row* rows, int n) {
   return PRC_D2Z_OK;
 }
 
+int prc_d2z_get_radius_diag(prc_d2z_ctx* c, prc_d2z_radius_diag* rows, int n) {
+  if (!c || !rows || n < c->p.Nscales) return c ? c->fail(PRC_D2Z_ERR_BAD_ARG, "table too small") : PRC_D2Z_ERR_BAD_ARG;
+  std::vector<prc_d2z_radius_row> base(c->p.Nscales);
+  const int rc = prc_d2z_get_radius_table(c, base.data(), c->p.Nscales);
+  if (rc) return rc;
+  for (int r = 0; r < c->p.Nscales; ++r) {
+    rows[r].R_mpch = base[r].R_mpch;
+    rows[r].sigma = base[r].sigma;
+    rows[r].sigma_grid = base[r].sigma_grid;
+    rows[r].A = c->rc[r].A;
+    // the kernels hold -inf where the reference's square root is NaN
+    rows[r].B = std::isinf(c->rc[r].B) ? NAN : c->rc[r].B;
+  }
+  return PRC_D2Z_OK;
+}
+
+// The scalar set-up pinned by the host: sigma_min and sigma(R) (the two GSL
+// QAGS results of delta2zreion_d2z.C:24-26,97-100) and erfizeta (the GSL Brent
+// result of :21); B_R is rebuilt from them as the set-up kernel builds it.
+int prc_d2z_set_scalar_overrides(prc_d2z_ctx* c, const float* sigma, const float* erfizeta) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  const int NS = c->p.Nscales;
+  if (sigma)
+    for (int r = 0; r <= NS; ++r)
+      if (!(sigma[r] >= 0.0f) || std::isinf(sigma[r]))
+        return c->fail(PRC_D2Z_ERR_BAD_ARG, "sigma override must be finite and >= 0 (entry " + std::to_string(r) + ")");
+  if (erfizeta && !std::isfinite(*erfizeta)) return c->fail(PRC_D2Z_ERR_BAD_ARG, "erfizeta override must be finite");
+  (void)hipSetDevice(c->p.device);
+  if (sigma) c->sc.sigma_min = sigma[0];
+  if (erfizeta) c->sc.erfizeta = *erfizeta;
+  std::vector<float> sig(NS);
+  for (int r = 0; r < NS; ++r) {
+    if (sigma) c->rc[r].sigma = sigma[r + 1];
+    const float smin = c->sc.sigma_min, s = c->rc[r].sigma;
+    const float dd = smin * smin - s * s;
+    c->rc[r].B = barrier_offset(dd, c->sc.erfizeta);
+    sig[r] = c->rc[r].sigma;
+  }
+  PRC_CHECK(c, hipMemcpyAsync(c->d_rc, c->rc.data(), sizeof(RadiusConst) * NS, hipMemcpyHostToDevice, c->stream),
+            "overrides");
+  PRC_CHECK(c, hipMemcpyAsync(c->d_sigma, sig.data(), sizeof(float) * NS, hipMemcpyHostToDevice, c->stream),
+            "overrides");
+  PRC_CHECK(c, hipMemcpyAsync(c->d_sc, &c->sc, sizeof(SetupScalars), hipMemcpyHostToDevice, c->stream), "overrides");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "overrides");
+  c->have_result = false;
+  return PRC_D2Z_OK;
+}
+
 int prc_d2z_history(prc_d2z_ctx* c, uint32_t* counts) {
   if (!c || !counts) return PRC_D2Z_ERR_BAD_ARG;
   if (!c->have_result) return c->fail(PRC_D2Z_ERR_STATE, "no result: call prc_d2z_run first");
@@ -839,8 +891,10 @@ int prc_d2z_bubble_mfp(prc_d2z_ctx* c, const float* history, int nskewers, uint6
     prev = zcur[ix] = history_redshift(history, x, prev, dzh, sorted);
   }
   // CellSize = BoxSize/N; bins of ln distance between ln CellSize and ln BoxSize
-  // (bubblehistory_d2z.C:95-101)
-  const float cell = c->p.BoxSize / (float)c->N;
+  // (bubblehistory_d2z.C:95-101), with BoxSize in Mpc: the d2z program divides
+  // the parameter file's Mpc/h by h before anything else (main_d2z.C:29)
+  const float box = box_mpc(c->p);
+  const float cell = box / (float)c->N;
   SkewerParams sp;
   sp.N = c->N;
   sp.nskewers = nskewers;
@@ -848,7 +902,7 @@ int prc_d2z_bubble_mfp(prc_d2z_ctx* c, const float* history, int nskewers, uint6
   sp.seed_hi = (unsigned)(seed >> 32);
   sp.cell = (double)cell;
   sp.lmin = logf(cell);
-  sp.lmax = (float)log(1.0 * (double)c->p.BoxSize);
+  sp.lmax = (float)log(1.0 * (double)box);
   sp.dmfp = (sp.lmax - sp.lmin) / (float)kNMfp;
   hipStream_t s = c->stream;
   PRC_CHECK(c, hipMemcpyAsync(c->d_zcur, zcur.data(), sizeof(float) * kNXMfp, hipMemcpyHostToDevice, s), "skewers");

@@ -224,6 +224,78 @@ void check(int rc, prc_d2z_ctx* ctx, const char* what) {
   if (rc != PRC_D2Z_OK) die("%s failed: %s", what, prc_d2z_last_error(ctx));
 }
 
+// GetHistory (bubblehistory_d2z.C:5-41): integer counts per bin from the GPU,
+// history[i] = count / N^3 in float, then the float running sum;
+// history_z[i] = 49.9 + i dzhistory.  The reference never sets history_z[0]
+// (its loop starts at 1); here it is 49.9.  Returns tau_es of the history.
+float get_history(prc_d2z_ctx* ctx, const Param& P, std::vector<float>& history, std::vector<float>& history_z) {
+  std::vector<uint32_t> counts(kNHistory);
+  check(prc_d2z_history(ctx, counts.data()), ctx, "history");
+  const float dzhistory = (float)((kHistoryFinal - kHistoryInitial) / (kNHistory - 1));
+  history.assign(kNHistory, 0.0f);
+  history_z.assign(kNHistory, 0.0f);
+  const double n3 = pow((double)P.N, 3);
+  for (int i = 0; i < kNHistory; ++i) history[i] = (float)((double)counts[i] / n3);
+  for (int i = 1; i < kNHistory; ++i) history[i] += history[i - 1];
+  history_z[0] = (float)kHistoryInitial;
+  for (int i = 1; i < kNHistory; ++i) history_z[i] = (float)(kHistoryInitial + i * dzhistory);
+  return tau_es(history_z, history, P.Omegam, P.Omegab, P.h);
+}
+
+// growth (cosmology.C:174-212) with the reference's float/double mixture:
+// float variables, the 2.5 and 4./7. and Ma et al. coefficients in double
+float growth_ref(float z, float omegam, float omegal, float w) {
+  const float x = 1 + z, x2 = x * x, x3 = x * x * x;
+  const float x3w = powf(x3, w);
+  const float omega = omegam * x3 / (omegam * x3 + (1 - omegam - omegal) * x2 + omegal);
+  const float lambda = omegal * x3 * x3w / (omegam * x3 + (1 - omegam - omegal) * x2 + omegal * x3 * x3w);
+  float g = (float)(2.5 * omega / (pow((double)omega, 4. / 7.) - lambda + (1 + omega / 2) * (1 + lambda / 70)));
+  float g0 = (float)(2.5 * omegam / (pow((double)omegam, 4. / 7.) - omegal + (1 + omegam / 2) * (1 + omegal / 70)));
+  const float t = (float)(-(0.255 + 0.305 * w + 0.0027 / w) * (1 - omega) -
+                          (0.366 + 0.266 * w - 0.0700 / w) * logf(omega));
+  const float t0 = (float)(-(0.255 + 0.305 * w + 0.0027 / w) * (1 - omegam) -
+                           (0.366 + 0.266 * w - 0.0700 / w) * logf(omegam));
+  g *= powf(-w, t);
+  g0 *= powf(-w, t0);
+  return (g / x) / g0;
+}
+
+// deltac (cosmology.C:141-172): the collapse threshold over D(z); the open
+// (Omega_Lambda = 0) and the flat fitting forms, float variables
+float deltac_ref(float z, float omegam, float omegal, float w) {
+  const float pi = 3.14159265358979f;
+  const float x = 1 + z, x2 = x * x, x3 = x * x * x;
+  const float omega = omegam * x3 / (omegam * x3 + (1 - omegam - omegal) * x2 + omegal);
+  const float eta = (float)log((2 / omega - 1) + sqrt(pow((double)(2 / omega - 1), 2) - 1));
+  float dc;
+  if ((omegam < 1) && (omegal == 0)) {
+    dc = (float)(1.5 * (3 * sinhf(eta) * (sinhf(eta) - eta) / pow((double)(coshf(eta) - 1), 2) - 2) *
+                 (1 + pow((double)(2 * pi / (sinhf(eta) - eta)), 2. / 3.)));
+  } else {
+    dc = (float)(3 * pow((double)(12 * pi), 2. / 3.) / 20 * (1 + 0.0123 * log10f(omega)));
+  }
+  dc /= growth_ref(z, omegam, omegal, w);
+  return dc;
+}
+
+// fcoll (cosmology.C:311-324): erfc(deltac / sqrt(2) / sigma_min), float erfc
+float fcoll_ref(float z, float omegam, float omegal, float w, float sigma_min) {
+  const float arg = (float)((double)deltac_ref(z, omegam, omegal, w) / sqrt(2.) / (double)sigma_min);
+  return erfcf(arg);
+}
+
+// Float2FloatTable::Float2Float (tables.C:27-38)
+float table_value(const std::vector<float>& t, float tmin, float tmax, float v) {
+  const int n = (int)t.size();
+  if (v <= tmin) return t[0];
+  if (v >= tmax) return t[n - 1];
+  const float delta = (tmax - tmin) / (float)n;
+  const int bin = (int)((v - tmin) / delta);
+  if (bin >= n - 1) return t[n - 1];  // the reference reads t[n] here
+  const float f = (v - (tmin + (float)bin * delta)) / delta;
+  return (1 - f) * t[bin] + f * t[bin + 1];
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -284,17 +356,10 @@ int main(int argc, char** argv) {
     printf("\n %10d %15.3f %15.3f %15.3f", r, rows[r].R_mpch, rows[r].sigma, rows[r].sigma_grid);
   printf("\n");
 
-  // GetHistory + tau (bubblehistory_d2z.C:5-41)
-  std::vector<uint32_t> counts(kNHistory);
-  check(prc_d2z_history(ctx, counts.data()), ctx, "history");
-  const float dzhistory = (float)((kHistoryFinal - kHistoryInitial) / (kNHistory - 1));
-  std::vector<float> history(kNHistory), history_z(kNHistory);
-  const double n3 = pow((double)P.N, 3);
-  for (int i = 0; i < kNHistory; ++i) history[i] = (float)((double)counts[i] / n3);
-  for (int i = 1; i < kNHistory; ++i) history[i] += history[i - 1];
-  history_z[0] = (float)kHistoryInitial;
-  for (int i = 1; i < kNHistory; ++i) history_z[i] = (float)(kHistoryInitial + i * dzhistory);
-  const float tau = tau_es(history_z, history, P.Omegam, P.Omegab, P.h);
+  // GetHistory of the uncorrected field (main_d2z.C:76): the input of the
+  // bubble statistics and of the correction
+  std::vector<float> history, history_z;
+  (void)get_history(ctx, P, history, history_z);
 
   // GetBubbleMFP (bubblehistory_d2z.C:82-242), WriteMFP and WriteMFPPDF
   // (io_d2z.C:343-400)
@@ -311,9 +376,11 @@ int main(int argc, char** argv) {
   {
     FILE* f = fopen((o.basename + ".mfppdf").c_str(), "wb");
     if (!f) die("cannot write %s.mfppdf", o.basename.c_str());
-    const float cell = P.BoxSize / (float)P.N;
+    // BoxSize in Mpc, as the reference holds it after main_d2z.C:29
+    const float box = P.BoxSize / P.h;
+    const float cell = box / (float)P.N;
     const int header1[2] = {PRC_D2Z_NX_MFP, PRC_D2Z_NMFP};
-    const float header2[4] = {0.01f, 0.999f, log10f(cell), (float)log10(1.0 * (double)P.BoxSize)};
+    const float header2[4] = {0.01f, 0.999f, log10f(cell), (float)log10(1.0 * (double)box)};
     if (fwrite(header1, sizeof(int), 2, f) != 2 || fwrite(header2, sizeof(float), 4, f) != 4 ||
         fwrite(dfdmfp.data(), sizeof(float), dfdmfp.size(), f) != dfdmfp.size())
       die("cannot write %s.mfppdf", o.basename.c_str());
@@ -322,13 +389,14 @@ int main(int argc, char** argv) {
 
   // GetCHistory and CorrectHistory (bubblehistory_d2z.C:44-80,301-343) with the
   // Float2FloatTable file (tables.C:9-25: float min, float max, int n, float[n])
+  std::vector<float> table;
+  float tmin = 0, tmax = 0;
   if (FILE* f = fopen(o.fcoll_table.c_str(), "rb")) {
-    float tmin = 0, tmax = 0;
     int n = 0;
     if (fread(&tmin, 4, 1, f) != 1 || fread(&tmax, 4, 1, f) != 1 || fread(&n, 4, 1, f) != 1 || n < 2 ||
         n > (1 << 28))
       die("malformed table %s", o.fcoll_table.c_str());
-    std::vector<float> table(n);
+    table.resize(n);
     if (fread(table.data(), 4, (size_t)n, f) != (size_t)n) die("short table %s", o.fcoll_table.c_str());
     fclose(f);
     std::vector<float> chistory(PRC_D2Z_NCHISTORY);
@@ -352,14 +420,26 @@ int main(int argc, char** argv) {
   }
   printf("\n Reionization redshifts written...");
 
-  // ionization history: z and the cumulative ionized fraction
+  // GetHistory of the written (corrected) field and its tau (main_d2z.C:97-98)
+  const float tau = get_history(ctx, P, history, history_z);
+  printf("\n\n tau = %f", tau);
+
+  // WriteHistory (io_d2z.C:284-314): z, x_HII(z), the fcoll table at z and
+  // fcoll(z) of the barrier's sigma_min.  The reference reads fcoll_table.tab
+  // here unconditionally; without it the third column is nan.
   {
+    prc_d2z_scalars sc;
+    check(prc_d2z_get_scalars(ctx, &sc), ctx, "scalars");
+    printf("\n Writing ionization history...");
     FILE* f = fopen((o.basename + ".history").c_str(), "w");
     if (!f) die("cannot write %s.history", o.basename.c_str());
-    for (int i = 0; i < kNHistory; ++i) fprintf(f, "%e %e\n", history_z[i], history[i]);
+    for (int i = 0; i < kNHistory; ++i) {
+      const float zi = history_z[i];
+      const float ft = table.empty() ? NAN : table_value(table, tmin, tmax, zi);
+      fprintf(f, "%e %e %e %e\n", zi, history[i], ft, fcoll_ref(zi, P.Omegam, P.Omegal, P.w, sc.sigma_min));
+    }
     fclose(f);
   }
-  printf("\n\n tau = %f", tau);
   printf("\n\n");
   prc_d2z_destroy(ctx);
   return 0;

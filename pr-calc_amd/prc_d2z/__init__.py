@@ -57,6 +57,11 @@ class CRadiusRow(C.Structure):
     _fields_ = [("R_mpch", C.c_float), ("sigma", C.c_float), ("sigma_grid", C.c_float)]
 
 
+class CRadiusDiag(C.Structure):
+    _fields_ = [("R_mpch", C.c_float), ("sigma", C.c_float), ("sigma_grid", C.c_float),
+                ("A", C.c_double), ("B", C.c_double)]
+
+
 class CTuning(C.Structure):
     _fields_ = [("radii_per_launch", C.c_int), ("z_blocks", C.c_int), ("ms_per_radius", C.c_double)]
 
@@ -76,6 +81,7 @@ EXPORTS = [
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
     "prc_d2z_bubble_mfp", "prc_d2z_corrected_history", "prc_d2z_correct_zreion",
+    "prc_d2z_get_radius_diag", "prc_d2z_set_scalar_overrides",
 ]
 
 _lib = None
@@ -129,6 +135,8 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_bubble_mfp": (C.c_int, [vp, fp, C.c_int, C.c_uint64, fp, fp, fp]),
         "prc_d2z_corrected_history": (C.c_int, [vp, fp, C.c_int, C.c_float, C.c_float, fp, fp]),
         "prc_d2z_correct_zreion": (C.c_int, [vp, fp, fp]),
+        "prc_d2z_get_radius_diag": (C.c_int, [vp, C.POINTER(CRadiusDiag), C.c_int]),
+        "prc_d2z_set_scalar_overrides": (C.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -245,6 +253,33 @@ class D2Z:
         rows = (CRadiusRow * n)()
         self._check(self.lib.prc_d2z_get_radius_table(self._ctx, rows, n))
         return np.array([(r.R_mpch, r.sigma, r.sigma_grid) for r in rows], dtype=np.float32)
+
+    def radius_diag(self) -> dict:
+        """The per-radius table with the barrier constants: arrays R_mpch, sigma,
+        sigma_grid (float32), A, B (float64; B is NaN where sigma(R) > sigma_min)."""
+        n = self.params.Nscales
+        rows = (CRadiusDiag * n)()
+        self._check(self.lib.prc_d2z_get_radius_diag(self._ctx, rows, n))
+        out = {f: np.array([getattr(r, f) for r in rows]) for f, _ in CRadiusDiag._fields_}
+        for f in ("R_mpch", "sigma", "sigma_grid"):
+            out[f] = out[f].astype(np.float32)
+        return out
+
+    def set_scalar_overrides(self, sigma=None, erfizeta=None):
+        """Pin sigma_min (sigma[0]), sigma(R_r) (sigma[1 + r]) and/or erfizeta to
+        given float values (e.g. the reference's GSL results)."""
+        sp = None
+        if sigma is not None:
+            arr = np.ascontiguousarray(sigma, dtype=np.float32)
+            if arr.shape != (self.params.Nscales + 1,):
+                raise ValueError(f"sigma must hold Nscales + 1 = {self.params.Nscales + 1} values")
+            self._sigma_override = arr
+            sp = arr.ctypes.data_as(C.c_void_p)
+        ep = None
+        if erfizeta is not None:
+            self._erfizeta_override = np.array([erfizeta], dtype=np.float32)
+            ep = self._erfizeta_override.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.prc_d2z_set_scalar_overrides(self._ctx, sp, ep))
 
     def history(self) -> np.ndarray:
         out = np.zeros(NHISTORY, dtype=np.uint32)

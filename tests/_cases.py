@@ -12,6 +12,11 @@ def box_for(N: int) -> float:
     return CELL * N
 
 
+def box_mpc(lp: D2ZParams) -> float:
+    """BoxSize in Mpc as the d2z program holds it (main_d2z.C:29, float division)."""
+    return float(np.float32(lp.BoxSize) / np.float32(lp.h))
+
+
 def lib_params(N, nscales=8, mode=1, rpl=0, rbub=0, rmax=None, zeta=40.0):
     box = box_for(N)
     return D2ZParams(N=N, BoxSize=box, Nscales=nscales, Rmax=rmax if rmax else box / 4, zeta=zeta,

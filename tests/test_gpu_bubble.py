@@ -15,7 +15,7 @@ import pytest
 
 import bubble as ob
 import d2z_fp64 as oracle
-from _cases import field, lib_params
+from _cases import box_mpc, field, lib_params
 from conftest import announce
 from prc_d2z import D2Z, D2ZError, D2ZParams, FUSED_FP32, FUSED_MIXED, NCHISTORY, NMFP, NX_MFP
 
@@ -45,7 +45,11 @@ def test_history_correction_vs_oracle(N, mode, nsk):
         z = d.zreion()
         hist = D2Z.cumulative_history(d.history(), N)
         mfp, mfp_x, dfd = d.bubble_mfp(hist, nskewers=nsk, seed=7)
-        mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, lp.BoxSize, nskewers=nsk, seed=7)
+        # the walks measure Mpc: the reference divides BoxSize by h first (main_d2z.C:29)
+        mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, box_mpc(lp), nskewers=nsk, seed=7)
+        mfp_h, *_ = ob.bubble_mfp(z, hist, lp.BoxSize, nskewers=nsk, seed=7)
+        assert not np.allclose(mfp, mfp_h, rtol=1e-3)  # Mpc/h units would differ by 1/h
+        np.testing.assert_allclose(mfp_h * np.float32(lp.h), mfp_o, rtol=1e-5)
         assert np.array_equal(mfp_x, x_o)
         assert dfd.shape == (NX_MFP, NMFP)
         assert np.abs(dfd.astype(np.int64) - dfd_o.astype(np.int64)).sum() <= 4
@@ -111,7 +115,7 @@ def test_history_correction_1024():
         z = d.zreion()
         hist = D2Z.cumulative_history(d.history(), N)
         mfp, mfp_x, dfd = d.bubble_mfp(hist, nskewers=nsk, seed=3)
-        mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, lp.BoxSize, nskewers=nsk, seed=3)
+        mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, box_mpc(lp), nskewers=nsk, seed=3)
         assert np.array_equal(mfp_x, x_o)
         assert np.abs(dfd.astype(np.int64) - dfd_o.astype(np.int64)).sum() <= 4
         np.testing.assert_allclose(mfp, mfp_o, rtol=1e-6)

@@ -59,6 +59,14 @@ def test_driver_end_to_end(tmp_path, mode):
     assert tau == pytest.approx(float(ref_tau), rel=tol_tau)
     used = (tmp_path / "param.d2z-usedvalues").read_text()
     assert "Nscales" in used and "DeltaFile" in used
+    # WriteHistory's four columns (io_d2z.C:284-314); no fcoll table here
+    hist = np.loadtxt(tmp_path / "output" / "run.history")
+    assert hist.shape == (oracle.NHISTORY, 4)
+    assert np.all(np.isnan(hist[:, 2]))
+    smin = float(g["scalars"][5])  # sigma_min of the oracle's set-up
+    zz = hist[1::997, 0]
+    np.testing.assert_allclose(hist[1::997, 3], [oracle.fcoll(z, 0.31, 0.69, -1.0, smin) for z in zz],
+                               rtol=1e-4, atol=1e-30)
 
 
 def test_driver_rejects_bad_parameter_file(tmp_path):
@@ -97,7 +105,9 @@ def test_driver_history_correction(tmp_path):
     raw = np.fromfile(tmp_path / "output" / "raw.zreion", dtype=np.float32).reshape(N, N, N)
     cor = np.fromfile(tmp_path / "output" / "cor.zreion", dtype=np.float32).reshape(N, N, N)
     hist = np.cumsum((oracle.history_counts(raw) / float(N) ** 3).astype(np.float32), dtype=np.float32)
-    mfp, mfp_x, dfd, _, _ = ob.bubble_mfp(raw, hist, box, nskewers=ob.NSKEWERS, seed=1)
+    # the walks measure Mpc: BoxSize/h (main_d2z.C:29)
+    box_mpc = float(np.float32(box) / np.float32(0.68))
+    mfp, mfp_x, dfd, _, _ = ob.bubble_mfp(raw, hist, box_mpc, nskewers=ob.NSKEWERS, seed=1)
     txt = np.loadtxt(tmp_path / "output" / "cor.mfp")
     assert txt.shape == (ob.NX_MFP, 2)
     np.testing.assert_allclose(txt[:, 0], mfp_x, atol=1e-6)
@@ -107,9 +117,27 @@ def test_driver_history_correction(tmp_path):
         h2 = np.fromfile(f, np.float32, 4)
         pdf = np.fromfile(f, np.float32)
     assert list(h1) == [ob.NX_MFP, ob.NMFP]
-    np.testing.assert_allclose(h2, [0.01, 0.999, np.log10(box / N), np.log10(box)], rtol=1e-6)
+    np.testing.assert_allclose(h2, [0.01, 0.999, np.log10(box_mpc / N), np.log10(box_mpc)], rtol=1e-6)
     assert pdf.size == ob.NX_MFP * ob.NMFP and np.abs(pdf - dfd.ravel()).sum() <= 4
     ch = ob.corrected_history(table, tmin, tmax, mfp, float(g["Rmax"]), float(g["zeta"]))
     ref = ob.correct_zreion(raw, hist, ch)
     assert np.max(np.abs(cor - ref)) < 1e-3
     assert not np.array_equal(cor, raw)
+    # main_d2z.C:97-101: GetHistory again on the corrected field, its tau
+    # printed, and WriteHistory's four columns from it
+    xh = oracle.history_float(oracle.history_counts(cor), N)
+    zh = oracle.history_z()
+    tau_cor = float(oracle.tau(zh, xh, 0.31, 0.049, 0.68))
+    tau_raw = float(oracle.tau(zh, oracle.history_float(oracle.history_counts(raw), N), 0.31, 0.049, 0.68))
+    tau = float(re.search(r"tau = ([\d.]+)", r1.stdout).group(1))
+    assert abs(tau_cor - tau_raw) > 1e-5  # the correction moves tau
+    assert tau == pytest.approx(tau_cor, abs=2e-6)
+    hcol = np.loadtxt(tmp_path / "output" / "cor.history")
+    assert hcol.shape == (ob.NHISTORY, 4)
+    np.testing.assert_allclose(hcol[:, 1], xh, rtol=1e-6, atol=1e-7)
+    idx = np.arange(1, ob.NHISTORY, 499)
+    np.testing.assert_allclose(hcol[idx, 2], [ob.table_value(table, tmin, tmax, z) for z in zh[idx]], rtol=1e-5,
+                               atol=1e-12)
+    smin = float(g["scalars"][5])
+    np.testing.assert_allclose(hcol[idx, 3], [oracle.fcoll(z, 0.31, 0.69, -1.0, smin) for z in zh[idx]],
+                               rtol=1e-4, atol=1e-30)

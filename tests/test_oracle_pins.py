@@ -156,6 +156,224 @@ def test_transeh_limits_and_baryon_suppression():
         assert a.transeh(kh * 0.8) == pytest.approx(b.transeh(kh * 0.5), rel=1e-6)
 
 
+# Eisenstein & Hu 1998 (ApJ 496, 605) define the no-wiggle transfer function
+# piece by piece; each test below pins one published piece of the oracle's
+# composite transeh through a limit or an inversion, so that a 1% change of any
+# constant in the oracle's formula fails at least one of them
+# (tests/mutate_oracle_constants.py runs that mutation check).
+E2 = 2 * math.e
+THETA2 = (2.728 / 2.7) ** 2          # eq. 28: q = k Theta_2.7^2 / Gamma (k in 1/Mpc)
+
+
+def _q_of(k, om, h, shape=None):
+    return (k / h) * THETA2 / (shape if shape is not None else om * h)
+
+
+def _T0(q):
+    """EH98 eqs. 29: L0 / (L0 + C0 q^2), C0 = 14.2 + 731 / (1 + 62.5 q)."""
+    L = math.log(E2 + 1.8 * q)
+    return L / (L + (14.2 + 731 / (1 + 62.5 * q)) * q * q)
+
+
+def _C_of(T, q):
+    """C0(q) recovered from T0 = L0 / (L0 + C0 q^2)."""
+    return (1 / T - 1) * math.log(E2 + 1.8 * q) / (q * q)
+
+
+def _shape_from_T(T, k, h):
+    """Invert T0 (monotone decreasing in q) by bisection: the effective shape
+    Gamma_eff with T = T0(q(k, Gamma_eff))."""
+    lo, hi = 1e-12, 1e12
+    for _ in range(400):
+        mid = math.sqrt(lo * hi)
+        if _T0(mid) > T:
+            lo = mid
+        else:
+            hi = mid
+        if hi / lo < 1 + 1e-15:
+            break
+    q = math.sqrt(lo * hi)
+    return (k / h) * THETA2 / q
+
+
+def test_transeh_large_q_asymptote():
+    """T0 -> L0 / (14.2 q^2) as q -> infinity (C0 -> 14.2), q from eq. 28 with
+    Gamma = Omega_m h when there are no baryons."""
+    om, h = float(np.float32(0.31)), float(np.float32(0.68))  # the reference's float parameters
+    eh = o._EH(om, 0.0, h, 0.965, 1.0)
+    for k in (1e5, 1e6, 1e7):
+        q = _q_of(k, om, h)
+        ratio = float(eh.transeh(k)) * 14.2 * q * q / math.log(E2 + 1.8 * q)
+        assert ratio == pytest.approx(1.0, abs=3e-6), (k, ratio)
+
+
+def test_transeh_small_q_limit():
+    """C0(q -> 0) = 14.2 + 731 = 745.2, and L0(0) = ln(2e)."""
+    om, h = float(np.float32(0.31)), float(np.float32(0.68))  # the reference's float parameters
+    eh = o._EH(om, 0.0, h, 0.965, 1.0)
+    k = 1e-6 * om * h * h / THETA2   # q = 1e-6
+    q = _q_of(k, om, h)
+    C = _C_of(float(eh.transeh(k)), q)
+    assert C == pytest.approx(745.2 * (1 - 62.5e-6 * 731 / 745.2), rel=2e-5)
+
+
+def test_transeh_crossover_of_C0():
+    """C0 is halfway between its limits, 14.2 + 731/2, at q = 1/62.5."""
+    om, h = float(np.float32(0.31)), float(np.float32(0.68))  # the reference's float parameters
+    eh = o._EH(om, 0.0, h, 0.965, 1.0)
+    q = 1 / 62.5
+    k = q * om * h * h / THETA2
+    assert _C_of(float(eh.transeh(k)), _q_of(k, om, h)) == pytest.approx(14.2 + 731 / 2, rel=1e-9)
+
+
+def _alpha_gamma(om, ob, h):
+    """EH98 eq. 31."""
+    wm, fb = om * h * h, ob / om
+    return 1 - 0.328 * math.log(431 * wm) * fb + 0.38 * math.log(22.3 * wm) * fb * fb
+
+
+def _sound_horizon(om, ob, h):
+    """EH98 eq. 26 (Mpc)."""
+    wm, wb = om * h * h, ob * h * h
+    return 44.5 * math.log(9.83 / wm) / math.sqrt(1 + 10 * wb ** 0.75)
+
+
+@pytest.mark.parametrize("om,ob,h", [(0.31, 0.049, 0.68), (0.25, 0.06, 0.72), (0.4, 0.02, 0.6)])
+def test_transeh_shape_limits_with_baryons(om, ob, h):
+    """EH98 eq. 30: Gamma_eff = Omega_m h (alpha + (1 - alpha) / (1 + (0.43 k s)^4)):
+    Omega_m h alpha at k s >> 1 (eq. 31), Omega_m h (1 + alpha)/2 where
+    0.43 k s = 1 and Omega_m h (alpha + (1 - alpha)/17) where it is 2 (s from
+    eq. 26).  The shapes are recovered by inverting T0."""
+    eh = o._EH(om, ob, h, 0.965, 1.0)
+    # float32 parameters, as the reference stores them
+    om32, ob32, h32 = (float(np.float32(v)) for v in (om, ob, h))
+    a = _alpha_gamma(om32, ob32, h32)
+    s = _sound_horizon(om32, ob32, h32)
+    base = om32 * h32
+    k_big = 1e4 / (0.43 * s)
+    assert _shape_from_T(float(eh.transeh(k_big)), k_big, h32) == pytest.approx(base * a, rel=1e-9)
+    k_t = 1 / (0.43 * s)
+    assert _shape_from_T(float(eh.transeh(k_t)), k_t, h32) == pytest.approx(base * (1 + a) / 2, rel=1e-9)
+    k_2 = 2 / (0.43 * s)
+    assert _shape_from_T(float(eh.transeh(k_2)), k_2, h32) == pytest.approx(base * (a + (1 - a) / 17), rel=1e-9)
+    assert eh.transeh(1e-7) == pytest.approx(1.0, abs=1e-5)
+
+
+def test_transeh_no_baryons_alpha_is_one():
+    """Omega_b = 0: alpha = 1 and Gamma_eff = Omega_m h at every k."""
+    om, h = 0.3, 0.7
+    eh = o._EH(om, 0.0, h, 0.965, 1.0)
+    for k in (1e-3, 0.05, 1.0, 30.0):
+        om32, h32 = float(np.float32(om)), float(np.float32(h))
+        assert _shape_from_T(float(eh.transeh(k)), k, h32) == pytest.approx(om32 * h32, rel=1e-9)
+
+
+# Ma, Caldwell, Bode & Wang 1999 (ApJ 521, L1): g(Omega_m, w) = g(Omega_m, -1)
+# (-w)^t, t = -(0.255 + 0.305 w + 0.0027/w)(1 - Omega_m)
+#             - (0.366 + 0.266 w - 0.07/w) ln Omega_m;
+# Carroll, Press & Turner 1992: g(Omega, Lambda) = 2.5 Omega / (Omega^(4/7) -
+# Lambda + (1 + Omega/2)(1 + Lambda/70)).  At high z, Omega(z) -> 1 and
+# t(z) -> 0, so the oracle's D(z)(1 + z) -> 1 / g0 with g0 = g(Omega_m, Omega_L)
+# (-w)^t0: both fits are recovered from the oracle's D and compared with the
+# published coefficients.
+def _inv_g0(om, ol, w):
+    z = 1e7  # Omega(z) = 1 - O(curvature / (Omega_m z)): float-exact only far out
+    return float(o.growth(z, om, ol, w)) * float(np.float32(np.float32(1) + np.float32(z)))
+
+
+def test_growth_cpt_coefficients_recovered():
+    """w = -1: 1/g0 = (Omega^p - Lambda + (1 + Omega/2)(1 + Lambda/c)) / (2.5 Omega);
+    recover p and 1/c by least squares over Omega_m, Omega_L (including open
+    and closed models, which the fitting form allows)."""
+    rows, rhs = [], []
+    for om in (0.1, 0.2, 0.3, 0.5, 0.7, 0.9):
+        for ol in (0.0, 0.3, 0.6, 0.9):
+            om32, ol32 = float(np.float32(om)), float(np.float32(ol))
+            y = _inv_g0(om, ol, -1.0) * 2.5 * om32 + ol32 - (1 + om32 / 2)
+            # y = Omega^p + (1 + Omega/2) Lambda / c: linear in 1/c given p
+            rows.append((om32, ol32, y))
+    best = None
+    for p in np.linspace(0.5, 0.65, 15001):
+        A = np.array([[(1 + r[0] / 2) * r[1]] for r in rows])
+        b = np.array([r[2] - r[0] ** p for r in rows])
+        inv_c, res, *_ = np.linalg.lstsq(A, b, rcond=None)
+        err = float(np.sum((A @ inv_c - b) ** 2))
+        if best is None or err < best[0]:
+            best = (err, p, float(inv_c[0]))
+    _, p, inv_c = best
+    assert p == pytest.approx(4 / 7, abs=2e-4)
+    assert inv_c == pytest.approx(1 / 70, rel=3e-3)
+
+
+def test_growth_ma_w_coefficients_recovered():
+    """t0 = ln(g0 / g_cpt) / ln(-w) over a grid of (Omega_m, w); the six
+    coefficients of the Ma et al. fit by least squares."""
+    A, b = [], []
+    for om in (0.1, 0.2, 0.3, 0.45, 0.6, 0.8):
+        for w in (-0.3, -0.4, -0.5, -0.6, -0.7, -1.4, -1.8, -2.5, -3.5):
+            om32, ol32, w32 = float(np.float32(om)), float(np.float32(1 - om)), float(np.float32(w))
+            g_cpt = 2.5 * om32 / (om32 ** (4 / 7) - ol32 + (1 + om32 / 2) * (1 + ol32 / 70))
+            g0 = 1 / _inv_g0(om, 1 - om, w)
+            t0 = math.log(g0 / g_cpt) / math.log(-w32)
+            lo = math.log(float(np.float32(om32)))
+            A.append([-(1 - om32), -(1 - om32) * w32, -(1 - om32) / w32, -lo, -lo * w32, -lo / w32])
+            b.append(t0)
+    coef, *_ = np.linalg.lstsq(np.array(A), np.array(b), rcond=None)
+    pub = np.array([0.255, 0.305, 0.0027, 0.366, 0.266, -0.07])
+    assert np.allclose(coef, pub, rtol=2e-3, atol=2e-5), coef
+
+
+def test_growth_w_fit_tracks_the_growth_equation_in_matter_normalisation():
+    """1/g0 is D(z -> inf)(1+z): compare the growth suppression g0 of the fit with
+    that of the wCDM growth equation (D = a early on) at w != -1 (a fitting
+    formula: agreement to 1.5%)."""
+    for om, w in ((0.31, -0.8), (0.25, -0.6), (0.35, -1.3)):
+        exact = _growth_exact(0.0, om, 1 - om, w) / _growth_exact(2000.0, om, 1 - om, w) / 2001.0
+        assert 1 / _inv_g0(om, 1 - om, w) == pytest.approx(exact, rel=0.015)
+
+
+def test_deltac_einstein_de_sitter():
+    """Omega_m = 1: delta_c = 3 (12 pi)^(2/3) / 20 = 1.686 times (1 + z)."""
+    dc0 = 3 * (12 * math.pi) ** (2 / 3) / 20
+    for z in (0.0, 6.0, 20.0):
+        assert float(o.deltac(z, 1.0, 0.0, -1.0)) == pytest.approx(dc0 * (1 + z), rel=3e-7)
+
+
+def test_deltac_open_branch_tends_to_eds():
+    """The open-universe form (Omega_L = 0, Omega_m < 1) tends to the EdS value as
+    Omega(z) -> 1 (high z); at z = 0 with Omega_m = 0.3 it is below 1.686."""
+    dc0 = 3 * (12 * math.pi) ** (2 / 3) / 20
+    # Omega(z = 50) = 0.956, where the O(eta^2) terms are ~1e-3; further out the
+    # reference's float sinh(eta) - eta cancels catastrophically
+    hi = float(o.deltac(50.0, 0.3, 0.0, -1.0)) * float(o.growth(50.0, 0.3, 0.0, -1.0))
+    assert hi == pytest.approx(dc0, rel=2e-3)
+    lo = float(o.deltac(0.0, 0.3, 0.0, -1.0))
+    assert 1.6 < lo < dc0
+
+
+def test_deltac_flat_omega_dependence_recovered():
+    """Flat models (Kitayama & Suto 1996): delta_c(Omega) = 1.686 (1 + 0.0123 log10
+    Omega); the slope is recovered from delta_c(z = 0) D(0) over Omega_m."""
+    dc0 = 3 * (12 * math.pi) ** (2 / 3) / 20
+    xs, ys = [], []
+    for om in (0.1, 0.2, 0.3, 0.5, 0.7, 0.9):
+        om32 = float(np.float32(om))
+        ol32 = float(np.float32(np.float32(1) - np.float32(om)))
+        xs.append(math.log10(om32))
+        ys.append(float(o.deltac(0.0, om32, ol32, -1.0)) / dc0 - 1)
+    slope = float(np.polyfit(xs, ys, 1)[0])
+    assert slope == pytest.approx(0.0123, rel=1e-3)
+
+
+def test_fcoll_is_erfc_of_the_threshold():
+    """fcoll = erfc(delta_c(z) / (sqrt 2 sigma_min)): 1 at sigma_min -> inf,
+    -> 0 at large z, and the closed form at one point (EdS)."""
+    dc0 = 3 * (12 * math.pi) ** (2 / 3) / 20
+    v = float(o.fcoll(3.0, 1.0, 0.0, -1.0, 5.0))
+    assert v == pytest.approx(math.erfc(dc0 * 4 / math.sqrt(2) / 5.0), rel=2e-6)
+    assert float(o.fcoll(40.0, 0.31, 0.69, -1.0, 5.6)) < 1e-6
+
+
 def test_sigma_integral_in_k_matches_ln_k():
     """sigma^2 = pnorm^2 int k^(2+ns) T^2 W^2 dk (the ln k Jacobian)."""
     eh = o._EH(0.31, 0.049, 0.68, 0.965, 1.0)
