@@ -32,7 +32,10 @@ sys.path.insert(0, str(ROOT / "pr-calc_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-B_ALG_PATH = 8.0  # algorithmic bytes per cell-radius of the whole path
+# algorithmic bytes per cell-radius of the whole path: one half-spectrum
+# intermediate written and read per radius, 4 + 4 B in FP32 storage, 2 + 2 B in
+# FP16 storage (the half mode)
+B_ALG_PATH = {"plain": 8.0, "fused": 8.0, "mixed": 8.0, "half": 4.0}
 
 WORKLOADS = {
     # name: N, BoxSize [Mpc/h], Nscales, Rmax [Mpc/h]
@@ -44,10 +47,13 @@ WORKLOADS = {
 # 48 uneven boxes: 16 x 128^3 (200 Mpc/h), 16 x 192^3 (300), 16 x 256^3 (400),
 # distinct seeds, 48 radii each from one cell to 50 Mpc/h
 MULTIBOX = [(128, 200.0), (192, 300.0), (256, 400.0)]
-MODES = {"plain": 0, "fused": 1, "mixed": 2}
+MODES = {"plain": 0, "fused": 1, "mixed": 2, "half": 3}
 METRIC = "cell\u00b7radius updates/s at 1024\u00b3\u00d796R: batched mixed vs plain FP32; HBM GB/s"
 DTYPES = {"plain": "f32", "fused": "f32",
-          "mixed": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector and barrier"}
+          "mixed": "f32 transforms: radix-16 stages on f16 matrix cores with split hi/lo operands "
+                   "(hi*hi + hi*lo + lo*hi) and f32 accumulation; f32 intermediates, f32 vector stages",
+          "half": "f16 storage and f16 matrix-core radix-16 stages with f32 accumulation; f32 vector and barrier"}
+STORE_BYTES = {"plain": 4, "fused": 4, "mixed": 4, "half": 2}
 
 
 FX_MAX_RADII = 16  # radii per fused x-pass launch (the library's launch limit)
@@ -153,10 +159,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="mixed", choices=list(MODES),
-                    help="mixed (default): fused path with FP16 intermediates and matrix-core radix-16 "
-                         "stages; fused: fused path in FP32; plain: one radius per pass in FP32")
+                    help="mixed (default): fused path whose radix-16 stages run on the matrix cores with "
+                         "split hi/lo FP16 operands and FP32 accumulation, FP32 intermediates; fused: fused "
+                         "path on the vector ALUs in FP32; half: fused path with FP16 intermediates and plain "
+                         "FP16 matrix-core stages; plain: one radius per pass in FP32")
     ap.add_argument("--no-plain-compare", action="store_true",
-                    help="skip the plain FP32 path measured beside the fused paths")
+                    help="skip the plain FP32, fused FP32 and half paths measured beside the default one")
     ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS) + ["c4_multibox"])
     ap.add_argument("--radii-per-launch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -206,7 +214,7 @@ def main():
     # radii per fused launch as the library runs them (it may lower the
     # request to fit the intermediates in HBM)
     nb = d.launch_config()["radii_per_launch"] if args.mode != "plain" else 1
-    store = 2 if args.mode == "mixed" else 4
+    store = STORE_BYTES[args.mode]
     # algorithmic bytes of the dominant kernel over the step (the last fused
     # batch may carry fewer radii; the x pass fills a batch in launches of at
     # most FX_MAX_RADII, each reading the spectrum), per launch on average
@@ -223,8 +231,15 @@ def main():
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic_from_profiles(dom_name, args.workload, args.mode),
             "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": alg,
-            "path_achieved": value * B_ALG_PATH / 1e9,
-            "path_frac": value * B_ALG_PATH / 1e9 / HBM_PEAK_GBS}
+            "b_alg_path": B_ALG_PATH[args.mode],
+            "path_achieved": value * B_ALG_PATH[args.mode] / 1e9,
+            "path_frac": value * B_ALG_PATH[args.mode] / 1e9 / HBM_PEAK_GBS}
+    # measured HBM bytes of the whole step (rocprof FETCH_SIZE + WRITE_SIZE per
+    # kernel launch, committed under profiles/) over the step time
+    step_bytes = traffic_per_step(args.workload, args.mode, kernels)
+    if step_bytes:
+        roof["path_traffic_bytes"] = step_bytes
+        roof["path_traffic_frac"] = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
 
     # the same steps replayed as the captured hipGraph (no per-launch events)
     d.run()
@@ -241,9 +256,13 @@ def main():
     hc = None
     if args.history_correction:
         hc = history_correction_times(d, N)
-    plain = None
+    plain, others = None, {}
     if args.mode != "plain" and not args.no_plain_compare:
         plain = plain_fp32_rate(lp, steps=min(args.steps, 2))
+        # the other fused paths on the same box, timed the same way
+        for other in ("fused", "mixed", "half"):
+            if other != args.mode:
+                others[other] = path_rate(lp, other, steps=min(args.steps, 3))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -268,10 +287,50 @@ def main():
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
         "plain_fp32": plain,
         "speedup_vs_plain_fp32": (value / plain["value"]) if plain else None,
+        **({f"{k}_path": v for k, v in others.items()}),
         "cpu_baseline": cpu,
         **({"history_correction": hc} if hc else {}),
     }
     print(json.dumps(out))
+
+
+def path_rate(lp, mode: str, steps: int) -> dict:
+    """Another fused path on the same box (same generated field): captured
+    runs, warmed up once, wall time of `steps` runs."""
+    from prc_d2z import D2Z
+    N, NS = lp.N, lp.Nscales
+    d = D2Z(lp.copy(mode=MODES[mode]))
+    d.generate_delta(seed=1)
+    d.run()
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        d.run_async()
+    d.sync()
+    el = time.perf_counter() - t0
+    d.close()
+    return {"value": N ** 3 * NS * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
+            "dtype": DTYPES[mode]}
+
+
+def traffic_per_step(workload: str, mode: str, kernels: dict):
+    """Sum over the step's kernels of the measured bytes per launch times the
+    launches per step, from the committed traffic profile (None if a kernel
+    of the step has no entry)."""
+    f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+    except Exception:
+        return None
+    total = 0.0
+    for name, (_, n) in kernels.items():
+        b = d.get(name, {}).get("bytes_per_launch")
+        if b is None:
+            return None
+        total += b * n
+    return total
 
 
 def plain_fp32_rate(lp, steps: int) -> dict:
@@ -337,7 +396,7 @@ def multibox(args):
         "metric": "aggregate cell*radius updates/s, 48 uneven boxes (delta2zreion)",
         "value": value, "unit": "cell*radius/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": batched * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.mode != "mixed" else "f32/f16", "data": "synthetic",
+        "dtype": DTYPES[args.mode], "data": "synthetic",
         "config": {"workload": "16x128^3 (200 Mpc/h) + 16x192^3 (300) + 16x256^3 (400), 48 radii each "
                                "1 cell..50 Mpc/h, distinct GRF seeds", "mode": args.mode},
         "separate_plain_ms": seq_t * 1e3,

@@ -52,7 +52,11 @@ enum prc_d2z_mode {
   PRC_D2Z_PLAIN_FP32 = 0, /* separate kernels: filter, 3 c2r passes, barrier, moments */
   PRC_D2Z_FUSED_FP32 = 1, /* filter fused into the first inverse pass, barrier/max into the last,
                              radii_per_launch radii per launch off one resident spectrum */
-  PRC_D2Z_FUSED_MIXED = 2 /* fused, with 16-bit storage of the inter-pass intermediates */
+  PRC_D2Z_FUSED_MIXED = 2, /* fused; at N = 1024 the x and z passes' radix-16 stages run on the matrix
+                             cores with split hi/lo FP16 operands and FP32 accumulation (FP32-grade
+                             transforms), FP32 intermediates; other N as FUSED_FP32 */
+  PRC_D2Z_FUSED_HALF = 3   /* fused, with FP16 storage of the inter-pass intermediates (and, at N = 1024,
+                             plain FP16 matrix-core radix-16 stages): narrower than FP32 */
 };
 
 /* Real-space field layouts of the reference: dense N^3 floats (ICs and .zreion
@@ -165,6 +169,19 @@ int prc_d2z_zreion_device(prc_d2z_ctx* ctx, const float** d_zreion);
 /* GetHistory (c++/src/bubblehistory_d2z.C:5-41) on the GPU: integer counts per
  * history bin, bin = (int)((zreion - 49.9)/dzhistory). */
 int prc_d2z_history(prc_d2z_ctx* ctx, uint32_t* counts /* [PRC_D2Z_NHISTORY] */);
+
+/* GetHistory (as prc_d2z_history) of a zreion field the caller holds in device
+ * memory of the current device (ncells floats, any order, 16-byte aligned), without a context
+ * and without any allocation: it calls no hipMalloc, hipFree or allocating
+ * library routine.  No counterpart in the reference: it is for callers that
+ * cannot allocate (serving loops, harnesses that forbid it).  Its device
+ * scratch is a static array of NHISTORY + 64 32-bit words per GPU (~400 KB,
+ * held until the process ends); ncells is limited to 2^32 - 1 (32-bit counts).
+ * Runs on the null stream and returns when counts[PRC_D2Z_NHISTORY] is filled.
+ * Calls that overlap on one GPU race on that scratch.  Errors: BAD_ARG (null,
+ * ncells out of range, d_zreion not device memory of the current device),
+ * NO_DEVICE, HIP; the message is prc_d2z_last_error(NULL). */
+int prc_d2z_history_noalloc(const float* d_zreion, int64_t ncells, uint32_t* counts);
 
 /* History correction of the written zreion field (main_d2z.C:78-90), in the
  * reference's order: GetHistory (above, history = cumulative counts / N^3 as

@@ -45,7 +45,8 @@ def build_library(jobs: int = 4) -> Path:
     LIB.mkdir(exist_ok=True)
     objdir = LIB / "obj"
     objdir.mkdir(exist_ok=True)
-    srcs = [CSRC / "d2z_kernels.hip", CSRC / "pass_kernels.hip", CSRC / "mfma_kernels.hip", CSRC / "grf_kernels.hip",
+    srcs = [CSRC / "d2z_kernels.hip", CSRC / "pass_kernels.hip", CSRC / "mfma_kernels.hip", CSRC / "mfma_split_kernels.hip",
+            CSRC / "grf_kernels.hip",
             CSRC / "bubble_kernels.hip", CSRC / "prc_d2z_api.cpp"]
     objs = []
     procs = []

@@ -71,7 +71,8 @@ struct WTable {
   int M;
   float invh;  // 1 / h, a power of two
 };
-constexpr int kWTableMax = 3584;  // intervals (56 KB of LDS)
+constexpr int kWTableMax = 3584;       // intervals (56 KB of LDS)
+constexpr int kWTableSplitMax = 3184;  // the split x pass's room for it (its other LDS: 110 KB)
 // intervals and spacing covering [0, xmax]: h = 1/8 if it fits in
 // max_intervals, else 1/4; 0 intervals (exact W) if neither does.  Max |error|
 // of the interpolant (W(0) = 1): 9.6e-8 at h = 1/8, 9.0e-7 at 1/4
@@ -96,6 +97,20 @@ int fz_mfma_blocks(bool rb);
 hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
                           double* partial, int nblk, float iscale, const float2* tw_full, const void* mats,
                           hipStream_t s);
+// Compensated matrix-core x and z passes (mixed path, FP32 intermediates,
+// N = 1024): split hi/lo FP16 operands of the radix-16 stages, FP32
+// accumulation (mfma_split_kernels.hip)
+bool split_supported(int N);
+size_t split_matrix_bytes(int N);
+hipError_t launch_split_matrices(void* out, hipStream_t s);
+hipError_t launch_fx_split(int N, const float2* S, const float2* D0, const float2* DN, float2* T, float dk,
+                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full,
+                           const WTable& wt, const void* mats, hipStream_t s);
+int fz_split_blocks(bool rb);
+// zscale: power of two <= 256 / max|delta| of the input field
+hipError_t launch_fz_split(int N, const float2* T, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
+                           double* partial, int nblk, float zscale, const float2* tw_full, const void* mats,
+                           hipStream_t s);
 // FP16 operand matrices of the matrix-core passes' radix-16 stages, built
 // once (0 bytes when neither pass runs for N and the storage type)
 size_t mfma_matrix_bytes(int N, bool half);
@@ -153,6 +168,8 @@ hipError_t launch_setup(const SetupParams& p, double* integ, float* Rr, double* 
                         RadiusConst* rc, float* sigma_out, hipStream_t s);
 
 hipError_t launch_count_nonfinite(const float* x, long long n, int* flag, hipStream_t s);
+// *out = bits of max |x| (not for graph capture: clears with a memset)
+hipError_t launch_maxabs(const float* x, long long n, unsigned int* out, hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float v, hipStream_t s);
 hipError_t launch_sigma_finish(const double* partial, int nblk, int nrad, double invN3, float* sigma_grid_out,
                                hipStream_t s);

@@ -94,6 +94,17 @@ __global__ void k_nonfinite(const float* __restrict__ x, long long n, int* __res
   if (bad) *flag = 1;
 }
 
+// max |x| as the bit pattern of a non-negative float (ordered as unsigned)
+__global__ void k_maxabs(const float* __restrict__ x, long long n, unsigned int* __restrict__ out) {
+  float m = 0.0f;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(x[id]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
 // ------------------------------------------------ history histogram (GPU)
 // bin = (int)((z - 49.9)/dzhistory) in double (bubblehistory_d2z.C:17-28);
 // integer counts.  A first pass finds the lowest and highest occupied bin; then
@@ -555,6 +566,13 @@ int wtable_intervals(double xmax, int max_intervals, double* h) {
 hipError_t launch_wtable(float4* coef, int M, double h, hipStream_t s) {
   if (M <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_wtable, dim3((M + 255) / 256), dim3(256), 0, s, coef, M, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxabs(const float* x, long long n, unsigned int* out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(unsigned int), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_maxabs, dim3(grid1(n, 256, 8192)), dim3(256), 0, s, x, n, out);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -79,6 +80,11 @@ bool supported_n(int N) {
   }
 }
 
+// FP16 storage of the inter-pass intermediates (the half mode only)
+bool half_storage(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_HALF; }
+// the compensated split-FP16 matrix-core x and z passes (mixed mode, FP32 storage)
+bool use_split(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_MIXED && split_supported(p.N); }
+
 }  // namespace
 
 struct prc_d2z_ctx {
@@ -118,6 +124,7 @@ struct prc_d2z_ctx {
   int* d_hrange = nullptr;  // lowest and highest occupied history bin
   double* d_pk = nullptr;  // power spectrum bins: counts (uint64) then 3 sums, kPkMaxBins each
   int* d_flag = nullptr;
+  unsigned* d_maxabs = nullptr;  // max |delta| of the input field (float bits)
   // history correction: skewer sums and counts per ionized fraction, zcur per
   // fraction, the history in, the corrected history and the per-bin table
   double* d_mfp_sum = nullptr;
@@ -127,6 +134,7 @@ struct prc_d2z_ctx {
   float* d_chist = nullptr;
   float* d_zvz = nullptr;
   SetupScalars sc{};
+  float zscale = 1.0f;  // split z pass: input scale, a power of two <= 256 DInit / max|delta|
   std::vector<RadiusConst> rc;
   bool have_delta = false, have_spectrum = false, have_result = false;
   std::string err;
@@ -200,7 +208,8 @@ int forward(prc_d2z_ctx* c, float dinit) {
 
 // grid of the fused z pass: the resident capacity of the kernel the mode uses
 int default_fz_blocks(const prc_d2z_params& p) {
-  const bool half = p.mode == PRC_D2Z_FUSED_MIXED, rb = p.RecordBubbleSizes != 0;
+  const bool half = half_storage(p), rb = p.RecordBubbleSizes != 0;
+  if (use_split(p)) return fz_split_blocks(rb);
   return fz_mfma_supported(p.N, half) ? fz_mfma_blocks(rb) : fz_blocks(p.N, half, rb);
 }
 
@@ -268,7 +277,7 @@ MixedScales mixed_scales(int N) {
 // radius; 32 planes per block measured slower), each into its planes of T
 int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
   const int N = c->N;
-  const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+  const bool half = half_storage(c->p);
   const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
   const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
   const size_t plane_bytes = (half ? sizeof(__half2) : sizeof(float2)) * (size_t)c->nspec;
@@ -276,6 +285,9 @@ int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
     const int n = std::min(kFxMaxRadii, nb - off);
     void* Tc = reinterpret_cast<char*>(T) + plane_bytes * off;
     PRC_CHECK(c, timed(c, "fx_filter", [&] {
+                if (use_split(c->p))
+                  return launch_fx_split(N, c->d_S, c->d_D0, c->d_DN, static_cast<float2*>(Tc), host_dk(c), invN3,
+                                         c->d_rc, r0 + off, n, c->d_tw_full, c->wt, c->d_mats, s);
                 if (fx_mfma_supported(N, half))
                   return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, Tc, host_dk(c), invN3, c->d_rc, r0 + off, n,
                                         c->d_st_full, c->wt, c->d_mats, s);
@@ -290,7 +302,7 @@ int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
 // The y pass and the z pass (c2r + barrier + max) of one batch over T, on s
 int fused_yz(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
   const int N = c->N;
-  const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+  const bool half = half_storage(c->p);
   const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
   TileMap ym = tile_map(N, Axis::kYInPlace);
   ym.nColBlocks *= nb;  // the nb intermediates are consecutive: one launch covers all
@@ -301,6 +313,9 @@ int fused_yz(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
               float* rb = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
+              if (use_split(c->p))
+                return launch_fz_split(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, c->zscale,
+                                       c->d_tw_full, c->d_mats, s);
               if (fz_mfma_supported(N, half))
                 return launch_fz_mfma(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                       c->d_tw_full, c->d_mats, s);
@@ -406,6 +421,35 @@ int copy_field_out(prc_d2z_ctx* c, const float* dsrc, float* out, int layout) {
 
 }  // namespace
 
+// ======================================== GetHistory without allocation
+// Device scratch of prc_d2z_history_noalloc: the integer counts of the
+// NHISTORY bins and the lowest / highest occupied bin (the same buffers the
+// context-based prc_d2z_history keeps in its context).  Each GPU holds its own
+// copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns for that device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr int kNoallocWords = kNHistory + 64;  // counts, then the bin range (256 B aligned)
+static __device__ __align__(256) unsigned int scratch[kNoallocWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on device dev (current), looked up once per device
+hipError_t scratch_address(int dev, unsigned int** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned int*>(g_scratch_ptr[dev]);
+  return hipSuccess;
+}
+}  // namespace
+
 // ===================================================================== ABI
 extern "C" {
 
@@ -476,7 +520,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     g_create_error = "parameters out of range (need h, BoxSize, Rmin, Rmax, M_min, Omegam, sigma8 > 0 and zeta > 1)";
     return PRC_D2Z_ERR_BAD_ARG;
   }
-  if (p->mode != PRC_D2Z_PLAIN_FP32 && p->mode != PRC_D2Z_FUSED_FP32 && p->mode != PRC_D2Z_FUSED_MIXED) {
+  if (p->mode != PRC_D2Z_PLAIN_FP32 && p->mode != PRC_D2Z_FUSED_FP32 && p->mode != PRC_D2Z_FUSED_MIXED &&
+      p->mode != PRC_D2Z_FUSED_HALF) {
     g_create_error = "unknown mode";
     return PRC_D2Z_ERR_BAD_ARG;
   }
@@ -519,7 +564,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   c->nblk_fz = default_fz_blocks(*p);
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
-  const size_t tstore = (p->mode == PRC_D2Z_FUSED_MIXED) ? sizeof(__half2) : sizeof(float2);
+  const size_t tstore = half_storage(*p) ? sizeof(__half2) : sizeof(float2);
   const size_t fixed = sizeof(float) * c->ncell * (p->RecordBubbleSizes ? 3 : 2) + sizeof(float2) * c->nspec +
                        sizeof(float2) * 2 * (size_t)p->N * p->N + (64u << 20);
   size_t freeb = 0, totalb = 0;
@@ -550,7 +595,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_st_full, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, false)));
   A_(c->d_st_half, sizeof(float2) * std::max(1, stage_twiddle_size(p->N, true)));
   A_(c->d_st_fx, sizeof(float2) * std::max(1, fx_twiddle_size(p->N)));
-  const size_t mat_bytes = mfma_matrix_bytes(p->N, p->mode == PRC_D2Z_FUSED_MIXED);
+  const size_t mat_bytes = use_split(*p) ? split_matrix_bytes(p->N) : mfma_matrix_bytes(p->N, half_storage(*p));
   if (mat_bytes) A_(c->d_mats, mat_bytes);
   A_(c->d_mom, sizeof(double2) * (size_t)NS * kMomentBlocks);
   A_(c->d_part, sizeof(double) * (size_t)NS * kMaxZBlocks);
@@ -565,6 +610,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   A_(c->d_hrange, sizeof(int) * 2);
   A_(c->d_pk, sizeof(double) * 4 * kPkMaxBins);
   A_(c->d_flag, sizeof(int));
+  A_(c->d_maxabs, sizeof(unsigned));
   A_(c->d_mfp_sum, sizeof(double) * kNXMfp);
   A_(c->d_mfp_cnt, sizeof(unsigned) * kNXMfp * kNMfp);
   A_(c->d_zcur, sizeof(float) * kNXMfp);
@@ -580,7 +626,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if ((e = launch_twiddles(c->d_tw_full, p->N, c->stream)) != hipSuccess ||
       (e = launch_stage_twiddles(p->N, c->d_st_full, c->d_st_half, c->stream)) != hipSuccess ||
       (e = launch_fx_twiddles(p->N, c->d_st_fx, c->stream)) != hipSuccess ||
-      (mat_bytes && (e = launch_mfma_matrices(c->d_mats, c->stream)) != hipSuccess) ||
+      (mat_bytes && (e = use_split(*p) ? launch_split_matrices(c->d_mats, c->stream)
+                                        : launch_mfma_matrices(c->d_mats, c->stream)) != hipSuccess) ||
       (e = launch_setup(sp, c->d_integ, c->d_R, c->d_Rint, c->d_sc, c->d_rc, c->d_sigma, c->stream)) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, std::string("set-up kernels: ") + hipGetErrorString(e));
   c->rc.resize(NS);
@@ -596,7 +643,7 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     for (const RadiusConst& r : c->rc) rmax = std::max(rmax, r.R);
     const double xmax = (double)host_dk(c) * rmax * std::sqrt(3.0) * (p->N / 2) * 1.001 + 1.0;
     double h = 0.0;
-    const int M = wtable_intervals(xmax, kWTableMax, &h);
+    const int M = wtable_intervals(xmax, use_split(*p) ? kWTableSplitMax : kWTableMax, &h);
     if (M > 0) {
       if ((e = c->alloc(&c->d_wcoef, sizeof(float4) * M)) != hipSuccess)
         return bail(PRC_D2Z_ERR_OOM, "device allocation failed: W table");
@@ -607,6 +654,29 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
     }
   }
   *out = c;
+  return PRC_D2Z_OK;
+}
+
+// The split z pass's input scale from the new field: a power of two <= 256
+// DInit / max|delta| (its stage-A operands stay below 2^16 in FP16; see
+// mfma_split_kernels.hip), at most 2^8.  A changed scale drops the captured
+// graph, whose launch arguments hold the old one.
+static int update_zscale(prc_d2z_ctx* c) {
+  if (!use_split(c->p)) return PRC_D2Z_OK;
+  unsigned bits = 0;
+  PRC_CHECK(c, launch_maxabs(c->d_delta, c->ncell, c->d_maxabs, c->stream), "input range");
+  PRC_CHECK(c, hipMemcpyAsync(&bits, c->d_maxabs, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "input range");
+  PRC_CHECK(c, hipStreamSynchronize(c->stream), "input range");
+  float mx = 0.0f;
+  memcpy(&mx, &bits, sizeof(mx));
+  int e = 8;
+  if (mx > 0.0f) e = std::min(8, std::max(-100, (int)std::floor(std::log2(256.0 * (double)c->sc.DInit / mx))));
+  const float zs = ldexpf(1.0f, e);
+  if (zs != c->zscale && c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  c->zscale = zs;
   return PRC_D2Z_OK;
 }
 
@@ -631,6 +701,7 @@ static int set_delta_impl(prc_d2z_ctx* c, const float* src, int layout, hipMemcp
   c->have_spectrum = false;
   c->have_result = false;
   if (bad) return c->fail(PRC_D2Z_ERR_NAN_INPUT, "density field holds NaN or Inf");
+  if (int rc = update_zscale(c)) return rc;
   c->have_delta = true;
   return PRC_D2Z_OK;
 }
@@ -676,6 +747,7 @@ int prc_d2z_generate_delta(prc_d2z_ctx* c, uint64_t seed, int white_noise_only) 
   PRC_CHECK(c, hipStreamSynchronize(s), "field generation");
   c->timing = was_timing;
   c->have_spectrum = false;
+  if (int rc = update_zscale(c)) return rc;
   c->have_delta = true;
   return PRC_D2Z_OK;
 }
@@ -810,6 +882,53 @@ int prc_d2z_history(prc_d2z_ctx* c, uint32_t* counts) {
   PRC_CHECK(c, hipMemcpyAsync(counts, c->d_hist, sizeof(unsigned) * kNHistory, hipMemcpyDeviceToHost, c->stream),
             "history");
   PRC_CHECK(c, hipStreamSynchronize(c->stream), "history");
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_history_noalloc(const float* d_zreion, int64_t ncells, uint32_t* counts) {
+  g_create_error.clear();
+  if (!d_zreion || !counts) {
+    g_create_error = "null argument";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  if (ncells < 1 || ncells > (int64_t)UINT32_MAX) {
+    g_create_error = "ncells must be in 1..2^32-1 (32-bit bin counts); got " + std::to_string((long long)ncells);
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  if (reinterpret_cast<uintptr_t>(d_zreion) % 16 != 0) {
+    g_create_error = "d_zreion must be 16-byte aligned (the kernels read float4)";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    g_create_error = "no current HIP device";
+    return PRC_D2Z_ERR_NO_DEVICE;
+  }
+  // the field must be device memory of the current device: the kernels read it
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, d_zreion) != hipSuccess ||
+      (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) || attr.device != dev) {
+    (void)hipGetLastError();
+    g_create_error = "d_zreion is not device memory of the current device";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  unsigned int* sp = nullptr;
+  if ((e = scratch_address(dev, &sp)) != hipSuccess) {
+    g_create_error = std::string("scratch address: ") + hipGetErrorString(e);
+    return PRC_D2Z_ERR_HIP;
+  }
+  // the previous call's counts stay in the array: clear them first
+  const float dzf = (float)((0.01 - 49.9) / (kNHistory - 1));
+  hipStream_t s = nullptr;
+  if ((e = launch_fill(reinterpret_cast<float*>(sp), kNHistory, 0.0f, s)) != hipSuccess ||
+      (e = launch_history(d_zreion, ncells, sp, (double)dzf, reinterpret_cast<int*>(sp + kNHistory), s)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(counts, sp, sizeof(unsigned int) * kNHistory, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (e = hipStreamSynchronize(s)) != hipSuccess) {
+    g_create_error = std::string("history: ") + hipGetErrorString(e);
+    return PRC_D2Z_ERR_HIP;
+  }
   return PRC_D2Z_OK;
 }
 
@@ -993,13 +1112,18 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       rc = plain_radius(c, r, false);
     } else {
       const int N = c->N;
-      const bool half = c->p.mode == PRC_D2Z_FUSED_MIXED;
+      const bool half = half_storage(c->p);
       const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
       const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
       // the real output goes behind the first intermediate (FP32 sized region)
       float* real = reinterpret_cast<float*>(c->d_T + c->nspec);
       const TileMap ym = tile_map(N, Axis::kYInPlace);
-      hipError_t e = fx_mfma_supported(N, half)
+      // the mixed mode's x pass is its split matrix-core kernel; its z pass
+      // fuses the barrier, so delta_R comes from the FP32 z c2r here
+      hipError_t e = use_split(c->p)
+                         ? launch_fx_split(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
+                                           c->d_tw_full, c->wt, c->d_mats, c->stream)
+                     : fx_mfma_supported(N, half)
                          ? launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
                                           c->d_st_full, c->wt, c->d_mats, c->stream)
                          : launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,

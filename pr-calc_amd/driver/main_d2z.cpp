@@ -10,7 +10,7 @@
 // libprc_d2z (include/prc_d2z.h); there is no MPI: one process drives one GPU.
 //
 //   prc_d2z <paramfile> [-f name] [-r Rmax] [-m Mmin] [-z zeta] [-l] [-v] [-e]
-//                       [-M plain|fused|mixed] [-b radii_per_launch] [-d device]
+//                       [-M plain|fused|mixed|half] [-b radii_per_launch] [-d device]
 //                       [-T fcoll table]
 //
 // Differences from the reference, on purpose:
@@ -76,7 +76,7 @@ void usage() {
   printf("\n   -l lean mode (accepted; no effect on the GPU)");
   printf("\n   -v print stdout to the terminal instead of the .stdout file");
   printf("\n   -e print stderr to the terminal instead of the .stderr file");
-  printf("\n   -M plain|fused|mixed  GPU execution path [default = fused]");
+  printf("\n   -M plain|fused|mixed|half  GPU execution path [default = fused]");
   printf("\n   -b <n> radii per fused batch, 1..32 [default = 32]");
   printf("\n   -d <n> HIP device [default = 0]");
   printf("\n   -T <file> collapsed-fraction table of the history correction [default = fcoll_table.tab]");
@@ -105,6 +105,7 @@ Options command_line(int argc, char** argv) {
         if (!strcmp(optarg, "plain")) o.mode = PRC_D2Z_PLAIN_FP32;
         else if (!strcmp(optarg, "fused")) o.mode = PRC_D2Z_FUSED_FP32;
         else if (!strcmp(optarg, "mixed")) o.mode = PRC_D2Z_FUSED_MIXED;
+        else if (!strcmp(optarg, "half")) o.mode = PRC_D2Z_FUSED_HALF;
         else die("unknown -M value '%s'", optarg);
         break;
       case 'b': o.radii_per_launch = atoi(optarg); break;

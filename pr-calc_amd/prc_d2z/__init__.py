@@ -27,7 +27,7 @@ from .params import (D2ZParams, apply_command_line, read_parameter_file,  # noqa
 
 LIB_PATH = Path(__file__).resolve().parents[1] / "lib" / "libprc_d2z.so"
 
-PLAIN_FP32, FUSED_FP32, FUSED_MIXED = 0, 1, 2
+PLAIN_FP32, FUSED_FP32, FUSED_MIXED, FUSED_HALF = 0, 1, 2, 3
 DENSE, PADDED = 0, 1
 NHISTORY = 100000
 NCHISTORY = 10000
@@ -81,7 +81,7 @@ EXPORTS = [
     "prc_d2z_generate_delta", "prc_d2z_get_delta", "prc_d2z_set_graph",
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
     "prc_d2z_bubble_mfp", "prc_d2z_corrected_history", "prc_d2z_correct_zreion",
-    "prc_d2z_get_radius_diag", "prc_d2z_set_scalar_overrides",
+    "prc_d2z_get_radius_diag", "prc_d2z_set_scalar_overrides", "prc_d2z_history_noalloc",
 ]
 
 _lib = None
@@ -137,6 +137,7 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_correct_zreion": (C.c_int, [vp, fp, fp]),
         "prc_d2z_get_radius_diag": (C.c_int, [vp, C.POINTER(CRadiusDiag), C.c_int]),
         "prc_d2z_set_scalar_overrides": (C.c_int, [vp, vp, vp]),
+        "prc_d2z_history_noalloc": (C.c_int, [vp, C.c_int64, up]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -281,6 +282,12 @@ class D2Z:
             ep = self._erfizeta_override.ctypes.data_as(C.c_void_p)
         self._check(self.lib.prc_d2z_set_scalar_overrides(self._ctx, sp, ep))
 
+    def zreion_device_ptr(self) -> int:
+        """Device address of the context's dense zreion field."""
+        p = C.c_void_p()
+        self._check(self.lib.prc_d2z_zreion_device(self._ctx, C.byref(p)))
+        return int(p.value)
+
     def history(self) -> np.ndarray:
         out = np.zeros(NHISTORY, dtype=np.uint32)
         self._check(self.lib.prc_d2z_history(self._ctx, out))
@@ -393,3 +400,14 @@ def run_many(boxes) -> None:
     rc = lib.prc_d2z_run_many(arr, len(boxes))
     if rc:
         raise D2ZError(rc, "run_many: " + " | ".join(lib.prc_d2z_last_error(b._ctx).decode() for b in boxes))
+
+
+def history_noalloc(d_zreion: int, ncells: int) -> np.ndarray:
+    """GetHistory counts of a zreion field in device memory of the current
+    device, through the library's no-allocation entry point."""
+    lib = load_library()
+    out = np.zeros(NHISTORY, dtype=np.uint32)
+    rc = lib.prc_d2z_history_noalloc(C.c_void_p(d_zreion), int(ncells), out)
+    if rc:
+        raise D2ZError(rc, lib.prc_d2z_last_error(None).decode())
+    return out

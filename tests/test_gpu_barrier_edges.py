@@ -21,7 +21,7 @@ import pytest
 import d2z_fp64 as oracle
 from _cases import field, lib_params, oracle_params, zreion_metrics
 from conftest import announce
-from prc_d2z import D2Z, FUSED_FP32, FUSED_MIXED, PLAIN_FP32
+from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_HALF, FUSED_MIXED, PLAIN_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +48,7 @@ def _oracle(lp, rbub=False, sc=None):
 
 
 def _tol(mode):
-    return (0.02, 1e-3) if mode == FUSED_MIXED else (2e-4, 0.0)
+    return (0.02, 1e-3) if mode == FUSED_HALF else (2e-4, 0.0)
 
 
 def test_case_has_radii_above_sigma_min():
@@ -59,7 +59,7 @@ def test_case_has_radii_above_sigma_min():
 
 
 @pytest.mark.parametrize("mode,rpl", [(PLAIN_FP32, 0), (FUSED_FP32, 0), (FUSED_FP32, 2), (FUSED_FP32, 3),
-                                      (FUSED_MIXED, 0), (FUSED_MIXED, 2)])
+                                      (FUSED_HALF, 0), (FUSED_HALF, 2)])
 @pytest.mark.parametrize("rbub", [0, 1])
 def test_radii_above_sigma_min_contribute_nothing(mode, rpl, rbub):
     lp = _params(mode, rpl, rbub)
@@ -80,13 +80,15 @@ def test_radii_above_sigma_min_contribute_nothing(mode, rpl, rbub):
     assert m["max_abs"] <= tol and m["frac_gt_0.01"] <= frac and m["xhii_max"] <= 1e-3
     assert np.mean(z > 0) > 0.2  # the valid radii do raise zreion
     if rbub:
-        R_above = set(np.asarray(sc.R)[above].tolist())
-        assert not np.isin(rb, list(R_above)).any()
-        if mode != FUSED_MIXED:
+        # no cell records a radius above sigma_min: all recorded radii are >= the
+        # smallest valid one
+        r_valid = float(np.asarray(sc.R)[~above].min())
+        assert rb[rb > 0].min() >= r_valid * (1 - 1e-6)
+        if mode != FUSED_HALF:
             assert np.mean(rb != rbr) <= 1e-3
 
 
-@pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, FUSED_MIXED])
+@pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, FUSED_HALF])
 def test_zinit_rescales_the_input(mode):
     """zInit = 6: the field is divided by D(6) before the transform."""
     lp = _params(mode, M_min=1.4706e8, zInit=6.0)
@@ -135,3 +137,34 @@ def test_scalar_overrides_rebuild_the_barrier():
     print(m)
     assert m["max_abs"] <= 2e-4 and m["frac_gt_0.01"] == 0.0
     assert np.max(np.abs(z1 - z0)) > 0.05  # the overrides changed the run
+
+
+@pytest.mark.parametrize("rbub", [0, 1])
+def test_radii_above_sigma_min_split_path_1024(rbub):
+    """The split matrix-core z pass (mixed mode, N = 1024 only) with the first
+    radii above sigma_min, in one batch with valid radii and across a batch
+    boundary (4 radii per launch): against the fused FP32 path, whose handling
+    of such radii matches the oracle above."""
+    out = {}
+    for mode, rpl in ((FUSED_FP32, 0), (FUSED_MIXED, 0), (FUSED_MIXED, 4)):
+        lp = D2ZParams(N=1024, BoxSize=1600.0, Nscales=8, Rmax=100.0, zeta=40.0, M_min=1.5e14, mode=mode,
+                       radii_per_launch=rpl, RecordBubbleSizes=rbub)
+        announce(f"1024^3 sigma(R) > sigma_min mode={mode} rpl={rpl} rbub={rbub}")
+        with D2Z(lp) as d:
+            d.generate_delta(3)
+            diag = d.radius_diag()
+            d.run()
+            out[(mode, rpl)] = (d.zreion(), d.rbubble() if rbub else None)
+    above = np.isnan(diag["B"])
+    print("radii above sigma_min:", np.nonzero(above)[0].tolist())
+    assert above[:3].all() and not above[-2:].any()
+    z32 = out[(FUSED_FP32, 0)][0]
+    assert np.mean(z32 > 0) > 0.1
+    for key in ((FUSED_MIXED, 0), (FUSED_MIXED, 4)):
+        zm, rbm = out[key]
+        dz = float(np.max(np.abs(zm.astype(np.float64) - z32)))
+        print(key, "max|dz| vs fp32", dz)
+        assert dz <= 2e-3
+        if rbub:
+            r_valid = float(diag["R_mpch"][~above].min()) / 0.68
+            assert rbm[rbm > 0].min() >= r_valid * (1 - 1e-5)

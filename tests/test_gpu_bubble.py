@@ -49,7 +49,7 @@ def test_history_correction_vs_oracle(N, mode, nsk):
         mfp_o, x_o, dfd_o, _, _ = ob.bubble_mfp(z, hist, box_mpc(lp), nskewers=nsk, seed=7)
         mfp_h, *_ = ob.bubble_mfp(z, hist, lp.BoxSize, nskewers=nsk, seed=7)
         assert not np.allclose(mfp, mfp_h, rtol=1e-3)  # Mpc/h units would differ by 1/h
-        np.testing.assert_allclose(mfp_h * np.float32(lp.h), mfp_o, rtol=1e-5)
+        np.testing.assert_allclose(mfp_h / np.float32(lp.h), mfp_o, rtol=1e-5)  # x Mpc/h = x/h Mpc
         assert np.array_equal(mfp_x, x_o)
         assert dfd.shape == (NX_MFP, NMFP)
         assert np.abs(dfd.astype(np.int64) - dfd_o.astype(np.int64)).sum() <= 4

@@ -1,7 +1,9 @@
-"""Mixed path (FP32 arithmetic, FP16 storage of the two inter-pass
-intermediates) against the FP64 oracle.
+"""The half path (FUSED_HALF: FP32 arithmetic, FP16 storage of the two
+inter-pass intermediates, FP16 matrix-core stages at N = 1024) against the
+FP64 oracle.  The compensated mixed path (FUSED_HALF) at 1024^3 is checked in
+test_gpu_parity.py::test_c3_size_1024 to FP32 tolerances.
 
-Tolerances (stated for the mixed path): max |dzreion| <= 0.02, the fraction of
+Tolerances (stated for the half path): max |dzreion| <= 0.02, the fraction of
 cells with |dzreion| > 0.01 at most 1e-3, x_HII(z) within 1e-3; the per-radius
 delta_R error is reported and bounded by 2e-3 of the field's rms (FP16 has a
 unit roundoff of 4.9e-4 and each value is rounded twice).
@@ -12,7 +14,7 @@ import pytest
 import d2z_fp64 as oracle
 from _cases import field, lib_params, oracle_params, zreion_metrics
 from conftest import announce
-from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_MIXED
+from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_HALF
 
 pytestmark = pytest.mark.gpu
 
@@ -20,9 +22,9 @@ pytestmark = pytest.mark.gpu
 # 192 runs the radix-3 plans (z c2r of length 96); every size but 1024 runs the
 # VALU z pass with its packed FP16 glue
 @pytest.mark.parametrize("N", [32, 64, 128, 192, 256])
-def test_mixed_zreion_vs_oracle(N):
-    lp = lib_params(N, nscales=10, mode=FUSED_MIXED)
-    announce(f"mixed zreion N={N}")
+def test_half_zreion_vs_oracle(N):
+    lp = lib_params(N, nscales=10, mode=FUSED_HALF)
+    announce(f"half zreion N={N}")
     with D2Z(lp) as d:
         d.set_delta(field(N))
         d.run()
@@ -39,9 +41,9 @@ def test_mixed_zreion_vs_oracle(N):
     np.testing.assert_allclose(tab[:, 2], sg, rtol=2e-3)
 
 
-def test_mixed_smoothed_field_error_per_radius():
+def test_half_smoothed_field_error_per_radius():
     N = 128
-    lp = lib_params(N, nscales=8, mode=FUSED_MIXED)
+    lp = lib_params(N, nscales=8, mode=FUSED_HALF)
     op = oracle_params(lp)
     sc = oracle.setup_scalars(op)
     dhat = oracle.forward(field(N), sc.DInit)
@@ -49,7 +51,7 @@ def test_mixed_smoothed_field_error_per_radius():
     with D2Z(lp) as d:
         d.set_delta(field(N))
         for r in range(lp.Nscales):
-            announce(f"mixed smoothed field N={N} r={r}")
+            announce(f"half smoothed field N={N} r={r}")
             g = d.smoothed_field(r).astype(np.float64)
             ref = oracle.smooth(dhat, kmag, sc.R[r])
             rms = np.sqrt(np.mean(ref ** 2))
@@ -61,10 +63,10 @@ def test_mixed_smoothed_field_error_per_radius():
 
 
 @pytest.mark.slow
-def test_mixed_against_fp32_at_512():
+def test_half_against_fp32_at_512():
     N = 512
     outs = {}
-    for mode in (FUSED_FP32, FUSED_MIXED):
+    for mode in (FUSED_FP32, FUSED_HALF):
         lp = D2ZParams(N=N, BoxSize=1000.0, Nscales=64, Rmax=80.0, zeta=40.0, M_min=1.4706e8, mode=mode)
         announce(f"512^3 mode={mode}")
         with D2Z(lp) as d:
@@ -72,22 +74,29 @@ def test_mixed_against_fp32_at_512():
             d.run()
             outs[mode] = (d.zreion(), d.history())
     z32, h32 = outs[FUSED_FP32]
-    z16, h16 = outs[FUSED_MIXED]
+    z16, h16 = outs[FUSED_HALF]
     dz = np.abs(z16.astype(np.float64) - z32)
     x32 = np.cumsum(h32.astype(np.int64)) / z32.size
     x16 = np.cumsum(h16.astype(np.int64)) / z16.size
-    print(f"512^3 mixed vs fp32: max|dz| {dz.max():.3e}  frac>0.01 {np.mean(dz > 0.01):.2e} "
+    print(f"512^3 half vs fp32: max|dz| {dz.max():.3e}  frac>0.01 {np.mean(dz > 0.01):.2e} "
           f"x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
     assert dz.max() <= 0.02
     assert np.max(np.abs(x32 - x16)) <= 1e-3
 
 
-def test_mixed_rbubble_1024():
-    """1024^3 with bubble radii recorded, first 8 radii: the mixed path's
-    zreion and rbubble against the fused FP32 path."""
+def test_half_rbubble_1024():
+    """1024^3 with bubble radii recorded, first 8 radii: the half path's
+    zreion and rbubble against the fused FP32 path.  A cell's recorded radius
+    is the first radius reaching the maximum of z(R) over radii; where z(R) is
+    flat over several radii (the field's value hardly changes from one
+    smoothing scale to the next, 4.4% apart), the FP16 path's error (~6e-3 in
+    z) moves that argmax by one or more radius steps.  The test bounds the
+    fraction of such cells and shows that the two paths' z values at a
+    mismatched cell agree to the path's tolerance: the radius moved, the
+    maximum did not."""
     N = 1024
     out = {}
-    for mode in (FUSED_FP32, FUSED_MIXED):
+    for mode in (FUSED_FP32, FUSED_HALF):
         lp = D2ZParams(N=N, BoxSize=2000.0, Nscales=8, Rmax=2000.0 / N * (120.0 / (2000.0 / N)) ** (7 / 95),
                        zeta=40.0, M_min=1.4706e8, mode=mode, RecordBubbleSizes=1)
         announce(f"1024^3 rbubble mode={mode}")
@@ -95,15 +104,21 @@ def test_mixed_rbubble_1024():
             d.generate_delta(1)
             d.run()
             out[mode] = (d.zreion(), d.rbubble())
-    (z32, rb32), (z16, rb16) = out[FUSED_FP32], out[FUSED_MIXED]
+    (z32, rb32), (z16, rb16) = out[FUSED_FP32], out[FUSED_HALF]
     dz = np.abs(z16.astype(np.float64) - z32)
     # neighbouring radii are 4.4% apart: where z(R) is flat over several radii
     # the winning radius moves with errors far below the mixed path's (about
     # 2% of cells at this size)
     radii = np.unique(np.concatenate([rb32[rb32 > 0], rb16[rb16 > 0]]))
-    mism = np.mean(rb16 != rb32)
-    print(f"1024^3 rbubble: mixed vs fp32 max|dz| {dz.max():.2e}, rbubble mismatch fraction {mism:.2e}")
+    mis = rb16 != rb32
+    mism = float(np.mean(mis))
+    idx = {float(r): i for i, r in enumerate(radii)}
+    steps = np.abs(np.vectorize(idx.get)(rb16[mis]) - np.vectorize(idx.get)(rb32[mis]))
+    hist = np.bincount(steps.astype(np.int64), minlength=8)
+    print(f"1024^3 rbubble: half vs fp32 max|dz| {dz.max():.2e}, rbubble mismatch fraction {mism:.2e}, "
+          f"radius steps of the mismatches (1..7): {hist[1:].tolist()}, max|dz| there {dz[mis].max():.2e}")
     assert radii.size <= 8
     assert dz.max() <= 0.02
     assert np.all((rb16 == 0) == (z16 == 0))
     assert mism <= 0.05
+    assert dz[mis].max() <= 0.02  # the maximum agrees where the radius moved
