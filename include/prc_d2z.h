@@ -246,14 +246,26 @@ int prc_d2z_power_spectrum(prc_d2z_ctx* ctx, int nk, float kmin, float kmax, dou
 int prc_d2z_set_launch_config(prc_d2z_ctx* ctx, int radii_per_launch, int z_blocks);
 int prc_d2z_get_launch_config(const prc_d2z_ctx* ctx, int* radii_per_launch, int* z_blocks, int* max_radii);
 
+/* FFT plans of the fused x and z passes: the vector radix plans (radix 2, 3,
+ * 4, 8, 16 stages on the vector ALUs) or the matrix-core plans (radix-16
+ * stages on the matrix cores: split hi/lo FP16 operands in the mixed mode,
+ * plain FP16 in the half mode; N = 1024).  Default: matrix-core where the
+ * mode has them.  x_radii_per_launch: radii per x-pass launch, 1..16. */
+enum prc_d2z_plan { PRC_D2Z_PLAN_VECTOR = 0, PRC_D2Z_PLAN_MATRIX = 1 };
+int prc_d2z_set_plan(prc_d2z_ctx* ctx, int x_plan, int z_plan, int x_radii_per_launch);
+int prc_d2z_get_plan(const prc_d2z_ctx* ctx, int* x_plan, int* z_plan, int* x_radii_per_launch);
+
 typedef struct {
   int radii_per_launch;
   int z_blocks;
   double ms_per_radius;
+  int x_plan, z_plan;     /* enum prc_d2z_plan */
+  int x_radii_per_launch;
 } prc_d2z_tuning;
 /* Autotuner: times the fused batches of the first radii for each candidate
- * launch shape on the current field, keeps and returns the fastest.  The host
- * persists the choice (see prc_d2z.tuning in the Python binding). */
+ * (x and z FFT plans, radii per x launch, radii per batch, z grid) on the
+ * current field, keeps and returns the fastest.  The host persists the choice
+ * (see prc_d2z.tuning in the Python binding). */
 int prc_d2z_autotune(prc_d2z_ctx* ctx, prc_d2z_tuning* out);
 
 /* Runs several contexts (boxes of any size on the same device) at once: every

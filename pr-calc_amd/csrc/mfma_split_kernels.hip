@@ -231,8 +231,7 @@ __global__ void __launch_bounds__(kFxThreads)
     for (int q = tid; q < ntab; q += kFxThreads) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, wscale, wsh, wM);
     __syncthreads();  // (1) W table complete; the previous radius's stage-B reads of bufX are done
     // stage A: filter, split, radix 16 on the matrix cores, outputs to bufX
-    // (rolled: one t's operands and accumulators at a time)
-#pragma unroll 1
+#pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int b = 4 * w + 2 * t + sel;
       float2 x[4];
@@ -245,7 +244,7 @@ __global__ void __launch_bounds__(kFxThreads)
           x[j] = cscale(m[4 * t + j], w0);
         } else {
           const float wn = wpk[fa];
-          const float2 dn = DN[(long long)kx * kN + ky];
+          const float2 dn = DN[(long long)(kx + zo) * kN + ky];  // address rebuilt per radius
           x[j] = make_float2(w0 * m[4 * t + j].x - wn * dn.y, w0 * m[4 * t + j].y + wn * dn.x);
         }
       }
@@ -261,8 +260,9 @@ __global__ void __launch_bounds__(kFxThreads)
     }
     __syncthreads();  // (2) stage A's outputs complete; its W table reads done
     // stage B: k = w; butterflies b' = w + 16 (2t + sel), inputs at b' + 64 r;
-    // outputs q at (2t + sel) 256 + w + 16 q stay in registers: Y[t][qi], q = g + 4 qi
-    float2 Y[2][4];
+    // outputs q = g + 4 qi of butterfly t at (2t + sel) 256 + w + 16 q stay in
+    // registers (named values, not an array: no stack copy)
+    float4_t yb[2][2];  // [t][block h]: outputs qi = 2h (d[0], d[1]) and 2h + 1 (d[2], d[3])
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int bp = w + 16 * (2 * t + sel);
@@ -270,40 +270,49 @@ __global__ void __launch_bounds__(kFxThreads)
 #pragma unroll
       for (int j = 0; j < 4; ++j) x[j] = bufX[xrow_s(bp + 64 * (4 * g + j)) * kCF + c];
       const SplitOp bop = split_b(x);
-      const float4_t d0 = mfma_split(gB0, bop);
-      const float4_t d1 = mfma_split(gB1, bop);
-      Y[t][0] = make_float2(d0[0], d0[1]);
-      Y[t][1] = make_float2(d0[2], d0[3]);
-      Y[t][2] = make_float2(d1[0], d1[1]);
-      Y[t][3] = make_float2(d1[2], d1[3]);
+      yb[t][0] = mfma_split(gB0, bop);
+      yb[t][1] = mfma_split(gB1, bop);
     }
     // stage C: radix 4, p = 256.  Butterfly b2 = w + 16 (g + 4 qi) takes
     // inputs r = 2t + sel' from this lane (sel' = sel) and lane l ^ 8; this
-    // lane computes qi = qbase + u, u < 2, and sends the partner its other two
-    float2 R[2][2];
+    // lane computes qi = qbase + u (u < 2): block h = sel of its own outputs,
+    // and sends the partner block h = 1 - sel.  The exchange is a DPP row
+    // rotation by 8 (lane l <-> l ^ 8 within each row of 16).
+    float4_t mine[2], recv[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
+      float4_t send;
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const float2 send = sel ? Y[t][u] : Y[t][2 + u];
-        R[t][u] = make_float2(__shfl_xor(send.x, 8, 64), __shfl_xor(send.y, 8, 64));
+      for (int i = 0; i < 4; ++i) {
+        mine[t][i] = sel ? yb[t][1][i] : yb[t][0][i];
+        send[i] = sel ? yb[t][0][i] : yb[t][1][i];
       }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        recv[t][i] = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(send[i]), 0x128 /* row_ror:8 */, 0xf, 0xf, false));
     }
     float2* dst = tp + (long long)rl * plane;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int qi = qbase + u;
       const int b2 = w + 16 * (g + 4 * qi);
+      const float2 y0 = make_float2(mine[0][2 * u], mine[0][2 * u + 1]);
+      const float2 y1 = make_float2(mine[1][2 * u], mine[1][2 * u + 1]);
+      const float2 p0 = make_float2(recv[0][2 * u], recv[0][2 * u + 1]);
+      const float2 p1 = make_float2(recv[1][2 * u], recv[1][2 * u + 1]);
       float2 v[4];
-      v[0] = sel ? R[0][u] : Y[0][qi];
-      v[1] = sel ? Y[0][qi] : R[0][u];
-      v[2] = sel ? R[1][u] : Y[1][qi];
-      v[3] = sel ? Y[1][qi] : R[1][u];
+      v[0] = sel ? p0 : y0;
+      v[1] = sel ? y0 : p0;
+      v[2] = sel ? p1 : y1;
+      v[3] = sel ? y1 : p1;
 #pragma unroll
       for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], s_twc[r * b2 + zo]);
       DFT<4, +1>::run(v);
+      // the stores' offsets are rebuilt from the opaque zero each radius (not
+      // hoisted out of the loop and held in registers)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) dst[(long long)(b2 + 256 * q) * kN * kC] = cscale(v[q], ibs);
+      for (int q = 0; q < 4; ++q) dst[(long long)(b2 + zo + 256 * q) * kN * kC] = cscale(v[q], ibs);
     }
   }
 }

@@ -84,6 +84,11 @@ bool supported_n(int N) {
 bool half_storage(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_HALF; }
 // the compensated split-FP16 matrix-core x and z passes (mixed mode, FP32 storage)
 bool use_split(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_MIXED && split_supported(p.N); }
+// a matrix-core plan exists for the mode and N: the split passes (mixed) or the
+// FP16 ones (half); otherwise both passes run their vector plans
+bool has_matrix_plan(const prc_d2z_params& p) {
+  return use_split(p) || (half_storage(p) && fx_mfma_supported(p.N, true) && fz_mfma_supported(p.N, true));
+}
 
 }  // namespace
 
@@ -112,6 +117,9 @@ struct prc_d2z_ctx {
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
   double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
   int nblk_fz = 0;
+  // FFT plans of the fused x and z passes (PRC_D2Z_PLAN_*) and radii per x launch
+  int xplan = PRC_D2Z_PLAN_VECTOR, zplan = PRC_D2Z_PLAN_VECTOR;
+  int fx_nb = kFxMaxRadii;
   float* d_sigma_grid = nullptr;
   float* d_sigma = nullptr;
   RadiusConst* d_rc = nullptr;
@@ -206,12 +214,14 @@ int forward(prc_d2z_ctx* c, float dinit) {
   return PRC_D2Z_OK;
 }
 
-// grid of the fused z pass: the resident capacity of the kernel the mode uses
-int default_fz_blocks(const prc_d2z_params& p) {
+// grid of the fused z pass: the resident capacity of the kernel the plan uses
+int default_fz_blocks(const prc_d2z_params& p, int zplan) {
   const bool half = half_storage(p), rb = p.RecordBubbleSizes != 0;
-  if (use_split(p)) return fz_split_blocks(rb);
-  return fz_mfma_supported(p.N, half) ? fz_mfma_blocks(rb) : fz_blocks(p.N, half, rb);
+  if (zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(p)) return use_split(p) ? fz_split_blocks(rb) : fz_mfma_blocks(rb);
+  return fz_blocks(p.N, half, rb);
 }
+bool matrix_x(const prc_d2z_ctx* c) { return c->xplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(c->p); }
+bool matrix_z(const prc_d2z_ctx* c) { return c->zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(c->p); }
 
 // BoxSize in Mpc as the d2z program holds it after Parameters.BoxSize /= h
 // (main_d2z.C:29, float division)
@@ -281,14 +291,14 @@ int fused_x(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
   const MixedScales ms = half ? mixed_scales(N) : MixedScales{1.0f, 1.0f, 1.0f};
   const float invN3 = (float)(1.0 / ((double)N * N * N)) * ms.s1;
   const size_t plane_bytes = (half ? sizeof(__half2) : sizeof(float2)) * (size_t)c->nspec;
-  for (int off = 0; off < nb; off += kFxMaxRadii) {
-    const int n = std::min(kFxMaxRadii, nb - off);
+  for (int off = 0; off < nb; off += c->fx_nb) {
+    const int n = std::min(c->fx_nb, nb - off);
     void* Tc = reinterpret_cast<char*>(T) + plane_bytes * off;
     PRC_CHECK(c, timed(c, "fx_filter", [&] {
-                if (use_split(c->p))
+                if (matrix_x(c) && use_split(c->p))
                   return launch_fx_split(N, c->d_S, c->d_D0, c->d_DN, static_cast<float2*>(Tc), host_dk(c), invN3,
                                          c->d_rc, r0 + off, n, c->d_tw_full, c->wt, c->d_mats, s);
-                if (fx_mfma_supported(N, half))
+                if (matrix_x(c))
                   return launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, Tc, host_dk(c), invN3, c->d_rc, r0 + off, n,
                                         c->d_st_full, c->wt, c->d_mats, s);
                 return launch_fx(N, c->d_S, c->d_D0, c->d_DN, Tc, half, host_dk(c), invN3, c->d_rc, r0 + off, n,
@@ -313,10 +323,10 @@ int fused_yz(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
             "inverse y pass");
   PRC_CHECK(c, timed(c, "fz_barrier", [&] {
               float* rb = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
-              if (use_split(c->p))
+              if (matrix_z(c) && use_split(c->p))
                 return launch_fz_split(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, c->zscale,
                                        c->d_tw_full, c->d_mats, s);
-              if (fz_mfma_supported(N, half))
+              if (matrix_z(c))
                 return launch_fz_mfma(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                       c->d_tw_full, c->d_mats, s);
               return launch_fz(N, T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
@@ -561,7 +571,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   // memory budget check before allocating
   const int NS = p->Nscales;
-  c->nblk_fz = default_fz_blocks(*p);
+  if (has_matrix_plan(*p)) c->xplan = c->zplan = PRC_D2Z_PLAN_MATRIX;
+  c->nblk_fz = default_fz_blocks(*p, c->zplan);
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
   const size_t tstore = half_storage(*p) ? sizeof(__half2) : sizeof(float2);
@@ -1120,10 +1131,10 @@ int prc_d2z_smoothed_field(prc_d2z_ctx* c, int r, float* out) {
       const TileMap ym = tile_map(N, Axis::kYInPlace);
       // the mixed mode's x pass is its split matrix-core kernel; its z pass
       // fuses the barrier, so delta_R comes from the FP32 z c2r here
-      hipError_t e = use_split(c->p)
+      hipError_t e = (matrix_x(c) && use_split(c->p))
                          ? launch_fx_split(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
                                            c->d_tw_full, c->wt, c->d_mats, c->stream)
-                     : fx_mfma_supported(N, half)
+                     : matrix_x(c)
                          ? launch_fx_mfma(N, c->d_S, c->d_D0, c->d_DN, c->d_T, host_dk(c), invN3, c->d_rc, r, 1,
                                           c->d_st_full, c->wt, c->d_mats, c->stream)
                          : launch_fx(N, c->d_S, c->d_D0, c->d_DN, c->d_T, half, host_dk(c), invN3, c->d_rc, r, 1,
@@ -1250,10 +1261,43 @@ int prc_d2z_get_launch_config(const prc_d2z_ctx* c, int* radii_per_launch, int* 
   return PRC_D2Z_OK;
 }
 
-// Times the fused batches covering the first min(Nscales, 8) radii for each
-// candidate (radii per launch in {cap, cap/2, .., 1} x z grid in {512 .. 4096})
-// with HIP events and keeps the fastest per radius.  Needs a density field;
-// leaves zreion undefined until the next run.
+int prc_d2z_set_plan(prc_d2z_ctx* c, int x_plan, int z_plan, int x_radii_per_launch) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  if (c->p.mode == PRC_D2Z_PLAIN_FP32) return c->fail(PRC_D2Z_ERR_STATE, "plans apply to fused modes");
+  for (int pl : {x_plan, z_plan}) {
+    if (pl != PRC_D2Z_PLAN_VECTOR && pl != PRC_D2Z_PLAN_MATRIX)
+      return c->fail(PRC_D2Z_ERR_BAD_ARG, "plan must be PRC_D2Z_PLAN_VECTOR or PRC_D2Z_PLAN_MATRIX");
+    if (pl == PRC_D2Z_PLAN_MATRIX && !has_matrix_plan(c->p))
+      return c->fail(PRC_D2Z_ERR_UNSUPPORTED_N, "no matrix-core plan for this mode and N (mixed and half, N = 1024)");
+  }
+  if (x_radii_per_launch < 1 || x_radii_per_launch > kFxMaxRadii)
+    return c->fail(PRC_D2Z_ERR_BAD_ARG, "x_radii_per_launch must be in 1.." + std::to_string(kFxMaxRadii));
+  if (z_plan != c->zplan) c->nblk_fz = default_fz_blocks(c->p, z_plan);
+  c->xplan = x_plan;
+  c->zplan = z_plan;
+  c->fx_nb = x_radii_per_launch;
+  if (c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_get_plan(const prc_d2z_ctx* c, int* x_plan, int* z_plan, int* x_radii_per_launch) {
+  if (!c) return PRC_D2Z_ERR_BAD_ARG;
+  if (x_plan) *x_plan = c->xplan;
+  if (z_plan) *z_plan = c->zplan;
+  if (x_radii_per_launch) *x_radii_per_launch = c->fx_nb;
+  return PRC_D2Z_OK;
+}
+
+// Times the fused batches covering the first min(Nscales, cap) radii for
+// each candidate and keeps the fastest per radius.  Candidates: the FFT plans
+// of the x and z passes (vector radix plans, and matrix-core radix-16 plans
+// where the mode has them), radii per x launch (8, 16), radii per batch (cap
+// and cap/2) and the z grid (the plan's resident capacity and twice it).  Every
+// candidate computes the same transforms; plans differ only in rounding.
+// Needs a density field; leaves zreion undefined until the next run.
 int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
   if (!c || !out) return PRC_D2Z_ERR_BAD_ARG;
   if (c->p.mode == PRC_D2Z_PLAIN_FP32) return c->fail(PRC_D2Z_ERR_STATE, "autotune applies to fused modes");
@@ -1267,56 +1311,61 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
     return rc;
   }
   const int nr = std::min(c->p.Nscales, c->nbatch_cap);
-  std::vector<int> nbs;
-  for (int nb = c->nbatch_cap; nb >= 1; nb = (nb > 1) ? nb / 2 : 0) nbs.push_back(nb);
-  // the resident-capacity grid chosen at create, its double, and fixed sizes
-  const int g0 = default_fz_blocks(c->p);
-  std::vector<int> grids = {g0, 512, 1024, 2048, 4096};
-  if (2 * g0 <= kMaxZBlocks) grids.push_back(2 * g0);
-  std::sort(grids.begin(), grids.end());
-  grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
+  std::vector<int> nbs = {c->nbatch_cap};
+  if (c->nbatch_cap >= 2) nbs.push_back(c->nbatch_cap / 2);
+  std::vector<int> plans = {PRC_D2Z_PLAN_VECTOR};
+  if (has_matrix_plan(c->p)) plans.push_back(PRC_D2Z_PLAN_MATRIX);
+  const std::vector<int> fxs = {8, kFxMaxRadii};
   hipEvent_t a = nullptr, b = nullptr;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
   double best = 1e30;
-  int best_nb = c->nbatch, best_grid = c->nblk_fz;
-  const int keep_nb = c->nbatch, keep_grid = c->nblk_fz;
-  for (int nb : nbs) {
-    for (int g : grids) {
-      c->nbatch = nb;
-      c->nblk_fz = g;
-      for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up
-        (void)hipEventRecord(a, c->stream);
-        for (int r0 = 0; r0 < nr && !rc; r0 += nb) {
-          const int n = std::min(nb, nr - r0);
-          if (!(rc = fused_x(c, r0, n, c->d_T, c->stream))) rc = fused_yz(c, r0, n, c->d_T, c->stream);
+  prc_d2z_tuning bt{c->nbatch, c->nblk_fz, 0.0, c->xplan, c->zplan, c->fx_nb};
+  const prc_d2z_tuning keep = bt;
+  for (int xp : plans)
+    for (int zp : plans)
+      for (int fx : fxs)
+        for (int nb : nbs) {
+          const int g0 = default_fz_blocks(c->p, zp);
+          for (int g : {g0, 2 * g0}) {
+            if (g > kMaxZBlocks || rc) continue;
+            c->xplan = xp;
+            c->zplan = zp;
+            c->fx_nb = fx;
+            c->nbatch = nb;
+            c->nblk_fz = g;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up
+              (void)hipEventRecord(a, c->stream);
+              for (int r0 = 0; r0 < nr && !rc; r0 += nb) {
+                const int n = std::min(nb, nr - r0);
+                if (!(rc = fused_x(c, r0, n, c->d_T, c->stream))) rc = fused_yz(c, r0, n, c->d_T, c->stream);
+              }
+              (void)hipEventRecord(b, c->stream);
+            }
+            if (rc) continue;
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            const double per = ms / nr;
+            if (per < best) {
+              best = per;
+              bt = prc_d2z_tuning{nb, g, per, xp, zp, fx};
+            }
+          }
         }
-        (void)hipEventRecord(b, c->stream);
-      }
-      if (rc) break;
-      (void)hipEventSynchronize(b);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, a, b);
-      const double per = ms / nr;
-      if (per < best) {
-        best = per;
-        best_nb = nb;
-        best_grid = g;
-      }
-    }
-    if (rc) break;
-  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   c->timing = was_timing;
   c->have_result = false;
-  c->nbatch = keep_nb;
-  c->nblk_fz = keep_grid;
+  c->xplan = keep.x_plan;
+  c->zplan = keep.z_plan;
+  c->fx_nb = keep.x_radii_per_launch;
+  c->nbatch = keep.radii_per_launch;
+  c->nblk_fz = keep.z_blocks;
   if (rc) return rc;
-  out->radii_per_launch = best_nb;
-  out->z_blocks = best_grid;
-  out->ms_per_radius = best;
-  return prc_d2z_set_launch_config(c, best_nb, best_grid);
+  *out = bt;
+  if ((rc = prc_d2z_set_plan(c, bt.x_plan, bt.z_plan, bt.x_radii_per_launch))) return rc;
+  return prc_d2z_set_launch_config(c, bt.radii_per_launch, bt.z_blocks);
 }
 
 int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n) {

@@ -63,7 +63,11 @@ class CRadiusDiag(C.Structure):
 
 
 class CTuning(C.Structure):
-    _fields_ = [("radii_per_launch", C.c_int), ("z_blocks", C.c_int), ("ms_per_radius", C.c_double)]
+    _fields_ = [("radii_per_launch", C.c_int), ("z_blocks", C.c_int), ("ms_per_radius", C.c_double),
+                ("x_plan", C.c_int), ("z_plan", C.c_int), ("x_radii_per_launch", C.c_int)]
+
+
+PLAN_VECTOR, PLAN_MATRIX = 0, 1
 
 
 class CTiming(C.Structure):
@@ -82,6 +86,7 @@ EXPORTS = [
     "prc_d2z_set_launch_config", "prc_d2z_get_launch_config", "prc_d2z_autotune", "prc_d2z_run_many",
     "prc_d2z_bubble_mfp", "prc_d2z_corrected_history", "prc_d2z_correct_zreion",
     "prc_d2z_get_radius_diag", "prc_d2z_set_scalar_overrides", "prc_d2z_history_noalloc",
+    "prc_d2z_set_plan", "prc_d2z_get_plan",
 ]
 
 _lib = None
@@ -138,6 +143,8 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_get_radius_diag": (C.c_int, [vp, C.POINTER(CRadiusDiag), C.c_int]),
         "prc_d2z_set_scalar_overrides": (C.c_int, [vp, vp, vp]),
         "prc_d2z_history_noalloc": (C.c_int, [vp, C.c_int64, up]),
+        "prc_d2z_set_plan": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "prc_d2z_get_plan": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -353,21 +360,32 @@ class D2Z:
         self._check(self.lib.prc_d2z_get_launch_config(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
         return {"radii_per_launch": a.value, "z_blocks": b.value, "max_radii": c.value}
 
+    def set_plan(self, x_plan: int, z_plan: int, x_radii_per_launch: int = 16):
+        """FFT plans of the fused x and z passes (PLAN_VECTOR / PLAN_MATRIX)."""
+        self._check(self.lib.prc_d2z_set_plan(self._ctx, x_plan, z_plan, x_radii_per_launch))
+
+    def plan(self) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.prc_d2z_get_plan(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return {"x_plan": a.value, "z_plan": b.value, "x_radii_per_launch": c.value}
+
     def autotune(self, cache: str | os.PathLike | None = None) -> dict:
-        """Pick the launch shape with the library's autotuner.  With ``cache`` (a
-        JSON file) a stored choice for this (device, N, mode) is reused, and a new
-        one is stored."""
+        """Pick FFT plans and launch shape with the library's autotuner.  With
+        ``cache`` (a JSON file) a stored choice for this (device, N, mode) is
+        reused, and a new one is stored."""
         import json
         key = f"{self.params.device}:N{self.N}:mode{self.params.mode}"
         if cache is not None and Path(cache).exists():
             table = json.loads(Path(cache).read_text())
             if key in table:
                 t = table[key]
+                self.set_plan(t["x_plan"], t["z_plan"], t["x_radii_per_launch"])
                 self.set_launch_config(t["radii_per_launch"], t["z_blocks"])
                 return dict(t, cached=True)
         t = CTuning()
         self._check(self.lib.prc_d2z_autotune(self._ctx, C.byref(t)))
-        res = {"radii_per_launch": t.radii_per_launch, "z_blocks": t.z_blocks, "ms_per_radius": t.ms_per_radius}
+        res = {"radii_per_launch": t.radii_per_launch, "z_blocks": t.z_blocks, "ms_per_radius": t.ms_per_radius,
+               "x_plan": t.x_plan, "z_plan": t.z_plan, "x_radii_per_launch": t.x_radii_per_launch}
         if cache is not None:
             table = json.loads(Path(cache).read_text()) if Path(cache).exists() else {}
             table[key] = res

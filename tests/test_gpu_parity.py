@@ -427,7 +427,9 @@ def test_autotune_and_launch_config(tmp_path):
         assert cfg0["radii_per_launch"] == 12 and cfg0["max_radii"] == 12  # all 12 radii in one batch
         cache = tmp_path / "tune.json"
         t = d.autotune(cache)
-        assert 1 <= t["radii_per_launch"] <= 12 and 512 <= t["z_blocks"] <= 4096 and not t["cached"]
+        assert 1 <= t["radii_per_launch"] <= 12 and 1 <= t["z_blocks"] <= 4096 and not t["cached"]
+        assert t["x_plan"] == t["z_plan"] == 0  # FP32 fused mode: vector plans only
+        assert t["x_radii_per_launch"] in (8, 16)
         d.run()
         assert np.array_equal(d.zreion(), z_default)   # launch shape never changes results
         assert d.autotune(cache)["cached"]
@@ -459,3 +461,40 @@ def test_run_many_matches_single_runs():
     for d, z in zip(boxes, ref):
         assert np.array_equal(d.zreion(), z)
         d.close()
+
+
+def test_autotune_plans_1024_mixed():
+    """1024^3, mixed mode: the autotuner chooses among the vector and
+    matrix-core FFT plans of the x and z passes as well as launch shapes.
+    Plans change speed, not results: every plan pair's zreion agrees with the
+    default's to the FP32-grade tolerance, and launch shapes within a plan
+    change nothing at all."""
+    from prc_d2z import D2ZParams, PLAN_MATRIX, PLAN_VECTOR
+    N = 1024
+    lp = D2ZParams(N=N, BoxSize=2000.0, Nscales=16, Rmax=2000.0 / N * (120.0 / (2000.0 / N)) ** (15 / 95),
+                   zeta=40.0, M_min=1.4706e8, mode=FUSED_MIXED)
+    announce("autotune plans 1024^3 mixed")
+    with D2Z(lp) as d:
+        d.generate_delta(1)
+        assert d.plan() == {"x_plan": PLAN_MATRIX, "z_plan": PLAN_MATRIX, "x_radii_per_launch": 16}
+        d.run()
+        z_default = d.zreion()
+        outs = {}
+        for xp in (PLAN_VECTOR, PLAN_MATRIX):
+            for zp in (PLAN_VECTOR, PLAN_MATRIX):
+                d.set_plan(xp, zp, 8)
+                d.run()
+                outs[(xp, zp)] = d.zreion()
+        t = d.autotune()
+        print("tuned:", t)
+        d.run()
+        z_tuned = d.zreion()
+        d.set_plan(t["x_plan"], t["z_plan"], 16 if t["x_radii_per_launch"] == 8 else 8)
+        d.run()
+        assert np.array_equal(d.zreion(), z_tuned)  # launch shape: bit for bit
+    assert np.array_equal(outs[(PLAN_MATRIX, PLAN_MATRIX)], z_default)
+    for k, z in outs.items():
+        dz = float(np.max(np.abs(z.astype(np.float64) - z_default)))
+        print(k, "max|dz| vs default plan", dz)
+        assert dz <= 2e-3
+    assert float(np.max(np.abs(z_tuned.astype(np.float64) - z_default))) <= 2e-3
