@@ -167,6 +167,11 @@ def main():
                     help="skip the plain FP32, fused FP32 and half paths measured beside the default one")
     ap.add_argument("--workload", default="c3_1024", choices=list(WORKLOADS) + ["c4_multibox"])
     ap.add_argument("--radii-per-launch", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the library's autotuner (FFT plans of the x and z passes, radii per x launch and "
+                         "per batch, z grid), which otherwise runs once before the warm-up")
+    ap.add_argument("--plan", default=None, metavar="X,Z",
+                    help="fixed FFT plans instead of tuning: vector or matrix for the x and z pass, e.g. matrix,vector")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--history", action="store_true",
@@ -193,6 +198,12 @@ def main():
                    mode=MODES[args.mode], radii_per_launch=args.radii_per_launch)
     d = D2Z(lp)
     d.generate_delta(seed=1)          # input resident in HBM before timing
+    tuning = None
+    if args.plan and args.mode != "plain":
+        xp, zp = (0 if v.strip() == "vector" else 1 for v in args.plan.split(","))
+        d.set_plan(xp, zp, FX_MAX_RADII)
+    elif not args.no_tune and args.mode != "plain":
+        tuning = d.autotune()        # untimed: plans and launch shape for this box
     for _ in range(args.warmup):
         d.run()
     d.set_timing(True)                # HIP events around every launch on the library's stream
@@ -214,6 +225,8 @@ def main():
     # radii per fused launch as the library runs them (it may lower the
     # request to fit the intermediates in HBM)
     nb = d.launch_config()["radii_per_launch"] if args.mode != "plain" else 1
+    plan = d.plan() if args.mode != "plain" else None
+    fx_nb = plan["x_radii_per_launch"] if plan else FX_MAX_RADII
     store = STORE_BYTES[args.mode]
     # algorithmic bytes of the dominant kernel over the step (the last fused
     # batch may carry fewer radii; the x pass fills a batch in launches of at
@@ -223,7 +236,7 @@ def main():
     else:
         launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
         if dom_name == "fx_filter":
-            launches = [min(FX_MAX_RADII, n - o) for n in launches for o in range(0, n, FX_MAX_RADII)]
+            launches = [min(fx_nb, n - o) for n in launches for o in range(0, n, fx_nb)]
         alg_step = sum(kernel_alg_bytes(dom_name, N, n, store) for n in launches)
     alg = alg_step / dom_n
     achieved = alg_step / (dom_ms * 1e-3) / 1e9 if alg_step else None
@@ -262,7 +275,7 @@ def main():
         # the other fused paths on the same box, timed the same way
         for other in ("fused", "mixed", "half"):
             if other != args.mode:
-                others[other] = path_rate(lp, other, steps=min(args.steps, 3))
+                others[other] = path_rate(lp, other, steps=min(args.steps, 3), tune=not args.no_tune)
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -281,7 +294,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{N}^3 box {box:g} Mpc/h, {NS} radii 1 cell..{rmax:g} Mpc/h, "
                                f"zeta 40, Mmin 1e8 Msun/h, GRF seed 1", "mode": args.mode,
-                   "radii_per_batch": nb, "x_pass_radii_per_launch": min(nb, FX_MAX_RADII)},
+                   "radii_per_batch": nb, "x_pass_radii_per_launch": min(nb, fx_nb),
+                   "fft_plans": ({"x": ("matrix" if plan["x_plan"] else "vector"),
+                                  "z": ("matrix" if plan["z_plan"] else "vector")} if plan else None),
+                   "autotuned": tuning is not None},
         "graph_ms_per_step": graph_ms,
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
@@ -294,13 +310,16 @@ def main():
     print(json.dumps(out))
 
 
-def path_rate(lp, mode: str, steps: int) -> dict:
-    """Another fused path on the same box (same generated field): captured
-    runs, warmed up once, wall time of `steps` runs."""
+def path_rate(lp, mode: str, steps: int, tune: bool = True) -> dict:
+    """Another fused path on the same box (same generated field): autotuned
+    like the main line, captured runs, warmed up once, wall time of `steps`
+    runs."""
     from prc_d2z import D2Z
     N, NS = lp.N, lp.Nscales
     d = D2Z(lp.copy(mode=MODES[mode]))
     d.generate_delta(seed=1)
+    if tune:
+        d.autotune()
     d.run()
     d.sync()
     t0 = time.perf_counter()
@@ -308,9 +327,11 @@ def path_rate(lp, mode: str, steps: int) -> dict:
         d.run_async()
     d.sync()
     el = time.perf_counter() - t0
+    plan = d.plan()
     d.close()
     return {"value": N ** 3 * NS * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
-            "dtype": DTYPES[mode]}
+            "dtype": DTYPES[mode], "fft_plans": {"x": "matrix" if plan["x_plan"] else "vector",
+                                                  "z": "matrix" if plan["z_plan"] else "vector"}}
 
 
 def traffic_per_step(workload: str, mode: str, kernels: dict):

@@ -213,6 +213,16 @@ template <> struct Plan<512> : PlanT<512, 8, 8, 8, 8> {};
 template <> struct Plan<1024> : PlanT<1024, 16, 16, 16, 4> {};
 template <> struct Plan<2048> : PlanT<2048, 16, 16, 16, 8> {};
 
+// Plans of the tiled column passes (forward and inverse x, y; the fused x
+// pass): 16 threads per pencil up to N = 256, so that a tile of 16 columns is
+// one 256-thread block for every N of that range (the multi-box kernels run
+// boxes of several sizes in one launch of 256-thread blocks).
+template <int L>
+struct TilePlan : Plan<L> {};
+template <> struct TilePlan<64> : PlanT<64, 4, 4, 4, 4> {};
+template <> struct TilePlan<128> : PlanT<128, 8, 8, 4, 4> {};
+template <> struct TilePlan<192> : PlanT<192, 12, 3, 4, 4, 4> {};
+
 // Wider plans for kernels that keep a second register copy of their tile
 // (the fused filter pass): fewer threads per pencil, more registers each.
 template <int L>

@@ -95,16 +95,21 @@ __device__ __forceinline__ void c2r_pre(float2 (&v)[P::E], const float2 (&m)[P::
 
 // ------------------------------------------------- forward z pass (r2c)
 // Dense real rows (x, y) -> packed half spectrum rows in the T layout.
+// LDS (float2) of one 256-thread block of the z-row kernels
+template <class P>
+constexpr int zrow_lds() {
+  return (256 / P::T) * LdsRow<P::L>::STRIDE;
+}
+
+// the 256 / T rows from row0 on (one block's work)
 template <class P, int C>
-__global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, float2* __restrict__ out,
-                                               long long nrows, float dinit, const float2* __restrict__ tw_half,
-                                               const float2* __restrict__ tw_full) {
+__device__ __forceinline__ void fwd_z_rows(const float* __restrict__ in, float2* __restrict__ out, long long nrows,
+                                           float dinit, const float2* __restrict__ tw_half,
+                                           const float2* __restrict__ tw_full, long long row0, float2* smem) {
   constexpr int L = P::L, T = P::T, E = P::E;
-  constexpr int ROWS = 256 / T;
-  __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
   const int t = threadIdx.x;
   const int i = t % T, rr = t / T;
-  const long long row = (long long)blockIdx.x * ROWS + rr;
+  const long long row = row0 + rr;
   const bool valid = row < nrows;
   const long long r = valid ? row : 0;
   const float2* src = reinterpret_cast<const float2*>(in + r * (2 * L));
@@ -123,20 +128,26 @@ __global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, flo
   }
 }
 
+template <class P, int C>
+__global__ void __launch_bounds__(256) k_fwd_z(const float* __restrict__ in, float2* __restrict__ out,
+                                               long long nrows, float dinit, const float2* __restrict__ tw_half,
+                                               const float2* __restrict__ tw_full) {
+  __shared__ float2 smem[zrow_lds<P>()];
+  fwd_z_rows<P, C>(in, out, nrows, dinit, tw_half, tw_full, (long long)blockIdx.x * (256 / P::T), smem);
+}
+
 // ------------------------------------------------- tiled complex pass
 // One block = one tile of C columns x L points along the transformed axis.
 // Tile t -> (o, cb) = (t mod nOuter, t / nOuter); element (a, c) is read at
 // cb*sCB + o*in_sO + a*in_sA + c and written at cb*sCB + o*out_sO + a*out_sA + c
 // (in place when in == out with equal strides).
 template <class P, int C, int DIR, class TI, class TO>
-__global__ void __launch_bounds__(P::T* C)
-    k_tile(const TI* in, TO* out, long long in_sA, long long in_sO, long long out_sA, long long out_sO,
-           long long sCB, int nOuter, float oscale, const float2* __restrict__ tw) {
-  constexpr int L = P::L, T = P::T, E = P::E;
-  __shared__ float2 smem[L * C];
+__device__ __forceinline__ void tile_body(const TI* in, TO* out, long long in_sA, long long in_sO, long long out_sA,
+                                          long long out_sO, long long sCB, int nOuter, float oscale,
+                                          const float2* __restrict__ tw, long long tile, float2* smem) {
+  constexpr int T = P::T, E = P::E;
   const int tid = threadIdx.x;
   const int c = tid % C, i = tid / C;
-  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   const long long o = tile % nOuter, cb = tile / nOuter;
   // running pointers: element i + e*T sits e*T*sA after element i
   const TI* ip = in + cb * sCB + o * in_sO + c + (long long)i * in_sA;
@@ -148,6 +159,15 @@ __global__ void __launch_bounds__(P::T* C)
   pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
 #pragma unroll
   for (int e = 0; e < E; ++e, op += ostep) st(op, 0, cscale(v[e], oscale));
+}
+
+template <class P, int C, int DIR, class TI, class TO>
+__global__ void __launch_bounds__(P::T* C)
+    k_tile(const TI* in, TO* out, long long in_sA, long long in_sO, long long out_sA, long long out_sO,
+           long long sCB, int nOuter, float oscale, const float2* __restrict__ tw) {
+  __shared__ float2 smem[P::L * C];
+  tile_body<P, C, DIR, TI, TO>(in, out, in_sA, in_sO, out_sA, out_sO, sCB, nOuter, oscale, tw,
+                               xcd_tile(blockIdx.x, gridDim.x), smem);
 }
 
 // Persistent in-place tile pass (the y passes; TS = float2 or __half2 storage):
@@ -372,30 +392,39 @@ __global__ void __launch_bounds__(256)
 // before the radius loop.  The table is laid out [|fx|][column] so that the
 // lanes of a wave (consecutive columns, then consecutive kx) read consecutive
 // words.
+// LDS of one fused x block: the padded exchange buffer (float2) and the |k|
+// table (float)
+template <class P, int CF>
+struct FxLds {
+  static constexpr int SH = ilog2(P::radices[0]);
+  static constexpr int NROW = P::L + (P::L >> SH);  // padded exchange rows
+  static constexpr int SMEM = NROW * CF;           // float2
+  static constexpr int KTAB = (CF + 1) * (P::L / 2 + 1);  // float
+};
+
+// One tile (ky, kz block, column sub-tile) of the fused x pass for nb radii;
+// wsh (the W(x) table, wM intervals) must be in LDS before the call.
 template <class P, int C, int CF, class TS>
-__global__ void __launch_bounds__(P::T* CF)
-    k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
-         TS* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
-         const float2* __restrict__ tw, const float4* __restrict__ wcoef, int wM, float winvh) {
+__device__ __forceinline__ void fx_body(const float2* __restrict__ S, const float2* __restrict__ D0,
+                                        const float2* __restrict__ DN, TS* __restrict__ Tbuf, float dk, float invN3,
+                                        const RadiusConst* __restrict__ rc, int r0, int nb,
+                                        const float2* __restrict__ tw, const float4* wsh, int wM, float winvh,
+                                        long long tile, float2* smem, float* ktab) {
   constexpr int N = P::L, T = P::T, E = P::E;
   constexpr int NSUB = C / CF;
   constexpr int NF = N / 2 + 1;           // distinct |fx|
   constexpr int NT = P::T * CF;           // threads
   constexpr long long Nh = N / 2;
   constexpr long long plane = (long long)N * N * Nh;
-  constexpr int SH = ilog2(P::radices[0]);
-  constexpr int NROW = N + (N >> SH);     // padded exchange rows
-  __shared__ float2 smem[NROW * CF];
+  constexpr int SH = FxLds<P, CF>::SH;
+  constexpr int NROW = FxLds<P, CF>::NROW;
   // W table [|fx|][CF] then the kz = N/2 column [|fx|]; it aliases the
   // transform's exchange buffer, which is idle between radii
   static_assert((CF + 1) * NF <= 2 * NROW * CF, "W table must fit in the exchange buffer");
   float* wtab = reinterpret_cast<float*>(smem);
   float* wpk = wtab + CF * NF;
-  __shared__ float ktab[(CF + 1) * NF];  // |k|/dk of the table entries, same order
-  extern __shared__ float4 wsh[];         // W(x) interpolation table (wM intervals)
   const int tid = threadIdx.x;
   const int cl = tid % CF, i = tid / CF;
-  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
   const int sub = (int)(tile % NSUB);
   const int ky = (int)((tile / NSUB) % N);
   const int kzb = (int)(tile / (NSUB * N));
@@ -425,7 +454,6 @@ __global__ void __launch_bounds__(P::T* CF)
     }
     ktab[q] = kmag_grid(f * f + fy2 + kzq * kzq);
   }
-  for (int q = tid; q < wM; q += NT) wsh[q] = wcoef[q];
   __syncthreads();  // ktab and the W(x) table complete
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
@@ -457,6 +485,19 @@ __global__ void __launch_bounds__(P::T* CF)
   }
 }
 
+template <class P, int C, int CF, class TS>
+__global__ void __launch_bounds__(P::T* CF)
+    k_fx(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+         TS* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0, int nb,
+         const float2* __restrict__ tw, const float4* __restrict__ wcoef, int wM, float winvh) {
+  __shared__ float2 smem[FxLds<P, CF>::SMEM];
+  __shared__ float ktab[FxLds<P, CF>::KTAB];  // |k|/dk of the table entries, same order
+  extern __shared__ float4 wsh[];              // W(x) interpolation table (wM intervals)
+  for (int q = threadIdx.x; q < wM; q += P::T * CF) wsh[q] = wcoef[q];
+  fx_body<P, C, CF, TS>(S, D0, DN, Tbuf, dk, invN3, rc, r0, nb, tw, wsh, wM, winvh, xcd_tile(blockIdx.x, gridDim.x),
+                        smem, ktab);
+}
+
 // columns per fused-x block: keep the block at <= 512 threads
 // Plan of the fused x pass.  It keeps a second register copy of its tile (the
 // unfiltered spectrum, reused for every radius), so at N = 1024 and 2048 it
@@ -465,7 +506,7 @@ __global__ void __launch_bounds__(P::T* CF)
 // of 8.
 template <int N>
 struct FxPlan {
-  using P = Plan<N>;
+  using P = TilePlan<N>;
 };
 template <>
 struct FxPlan<1024> {
@@ -496,36 +537,52 @@ struct FxCols {
 // delta_R^2 (float products, double sums as GetMeanSigma; with FP16 storage
 // the wave's own sum is float) are kept per wave in
 // LDS and written as one partial per block and radius.
-template <class P, int C, bool RB, class TS>
-__global__ void __launch_bounds__(256)
-    k_fz(const TS* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
-         const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
-         const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
-  constexpr int L = P::L, T = P::T, E = P::E;
-  constexpr int ROWS = 256 / T;
-  __shared__ float2 smem[ROWS * LdsRow<L>::STRIDE];
-  __shared__ double red[4][kFusedMaxRadii];
-  // twiddles in LDS (the blocks are persistent: copied once), the plan's
-  // stages transposed so that lanes with consecutive k read consecutive words
-  constexpr int NTW = plan_tw_size<P>() > 0 ? plan_tw_size<P>() : 1;
-  // c2r glue: w^k for FP32 storage, the packed FP16 constants (P, Q) of
-  // c2r_glue_h2 for FP16 storage
+// LDS of one fused z block: the pencil rows, the per-wave sums, the plan's
+// twiddles (transposed) and the c2r glue constants
+template <class P>
+struct FzLds {
+  static constexpr int NTW = plan_tw_size<P>() > 0 ? plan_tw_size<P>() : 1;
+  static constexpr int SMEM = (256 / P::T) * LdsRow<P::L>::STRIDE;  // float2
+};
+
+// Fills the z block's LDS tables: red = 0, the plan's twiddles transposed, w^k
+// (FP32 storage) or the packed FP16 glue constants (FP16 storage).  A barrier
+// must follow before fz_group reads them.
+template <class P, class TS>
+__device__ __forceinline__ void fz_tables(double (*red)[kFusedMaxRadii], float2* s_twh, float2* s_twf,
+                                          const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
+  constexpr int L = P::L;
   constexpr bool kH = std::is_same<TS, __half2>::value;
-  __shared__ float2 s_twh[NTW];
-  __shared__ float2 s_twf[L];
-  uint2* const s_pq = reinterpret_cast<uint2*>(s_twf);
   const int t = threadIdx.x;
-  const int i = t % T, rr = t / T;
-  const int w = t >> 6, lane = t & 63;
   if (t < 4 * kFusedMaxRadii) red[t / kFusedMaxRadii][t % kFusedMaxRadii] = 0.0;
   copy_twiddles_transposed<P>(s_twh, tw_half, t, 256);
+  uint2* const s_pq = reinterpret_cast<uint2*>(s_twf);
   for (int q = t; q < L; q += 256) {
     if constexpr (kH)
       s_pq[q] = c2r_glue_pq(q, tw_full[q]);
     else
       s_twf[q] = tw_full[q];
   }
-  __syncthreads();
+}
+
+// One group of 256 / T z rows from base on, for the nb radii of a batch: each
+// row's intermediates are transformed in turn, d2z() is evaluated per cell,
+// the maximum over the batch stays in registers (first radius wins ties) and
+// zreion (and rbubble) is read and written once.  Per-radius sums of
+// delta_R^2 (float products, double sums as GetMeanSigma; with FP16 storage
+// the wave's own sum is float) are added per wave into red.
+template <class P, int C, bool RB, class TS>
+__device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __restrict__ zre,
+                                         float* __restrict__ rbub, long long nrows,
+                                         const RadiusConst* __restrict__ rc, int r0, int nb, float iscale,
+                                         long long base, const float2* s_twh, const float2* s_twf, float2* smem,
+                                         double (*red)[kFusedMaxRadii]) {
+  constexpr int L = P::L, T = P::T, E = P::E;
+  constexpr bool kH = std::is_same<TS, __half2>::value;
+  const uint2* const s_pq = reinterpret_cast<const uint2*>(s_twf);
+  const int t = threadIdx.x;
+  const int i = t % T, rr = t / T;
+  const int w = t >> 6, lane = t & 63;
   const long long plane = nrows * L;
   // FP16 storage with 8-element pencils (N <= 1024): the pass is issue-bound
   // and each wave synchronises its own pencil.  Otherwise block barriers keep
@@ -535,94 +592,115 @@ __global__ void __launch_bounds__(256)
   using Lds = LdsRow<L, kH && (E <= 8)>;
   const Lds lds{smem + rr * Lds::STRIDE};
   const double sc = (double)iscale, sc2 = sc * sc;
+  const long long row = base + rr;
+  const bool valid = row < nrows;
+  const long long r = valid ? row : 0;
+  float2 best[E];
+  float bR0[E], bR1[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    best[e] = make_float2(0.0f, 0.0f);  // zreion >= 0: see zmax_nonneg
+    bR0[e] = 0.f;
+    bR1[e] = 0.f;
+  }
 #pragma unroll 1
-  for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS) {
-    const long long row = base + rr;
-    const bool valid = row < nrows;
-    const long long r = valid ? row : 0;
-    float2 best[E];
-    float bR0[E], bR1[E];
+  for (int q = 0; q < nb; ++q) {
+    const TS* tp = Tbuf + (long long)q * plane;
+    float2 v[E];
+    // read per radius from LDS rather than held in registers across radii
+    int zero = 0;
+    asm volatile("" : "+s"(zero));
+    if constexpr (kH) {  // glue on the stored FP16 pairs: one move and four packed FMAs each
+      const uint32_t* tu = reinterpret_cast<const uint32_t*>(tp);
+      uint32_t a[E], m[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) a[e] = tu[zrow_offset<P, C>(i, e, r, nrows)];
+      mirror_lanes<P>(a, m, i);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const uint2 pq = s_pq[zero + i + e * T];
+        v[e] = unpack_h2(c2r_glue_h2(a[e], m[e], pq.x, pq.y));
+      }
+    } else {
+      float2 m[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+      mirror_lanes<P>(v, m, i);
+      c2r_pre<P>(v, m, i, s_twf + zero);
+    }
+    pencil_fft<P, +1, 0, Lds, true>(v, i, s_twh + zero, lds);
+    // the output scale iscale (a power of two) is applied inside the
+    // barrier's FMA and to the sum of squares, where it is exact
+    // (the FP16-storage path evaluates it in FP32, far inside its own error)
+    const double A = rc[r0 + q].A, B = rc[r0 + q].B;
+    const BarrierF32 bk = barrier_f32(A, B, iscale);
+    const float R = rc[r0 + q].R;
+    float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      best[e] = make_float2(0.0f, 0.0f);  // zreion >= 0: see zmax_nonneg
-      bR0[e] = 0.f;
-      bR1[e] = 0.f;
-    }
-#pragma unroll 1
-    for (int q = 0; q < nb; ++q) {
-      const TS* tp = Tbuf + (long long)q * plane;
-      float2 v[E];
-      // read per radius from LDS rather than held in registers across radii
-      int zero = 0;
-      asm volatile("" : "+s"(zero));
-      if constexpr (kH) {  // glue on the stored FP16 pairs: one move and four packed FMAs each
-        const uint32_t* tu = reinterpret_cast<const uint32_t*>(tp);
-        uint32_t a[E], m[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e) a[e] = tu[zrow_offset<P, C>(i, e, r, nrows)];
-        mirror_lanes<P>(a, m, i);
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const uint2 pq = s_pq[zero + i + e * T];
-          v[e] = unpack_h2(c2r_glue_h2(a[e], m[e], pq.x, pq.y));
+      const float z0 = kH ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
+      const float z1 = kH ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
+      acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
+      if constexpr (RB) {
+        if (z0 > best[e].x) {
+          best[e].x = z0;
+          bR0[e] = R;
         }
-      } else {
-        float2 m[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
-        mirror_lanes<P>(v, m, i);
-        c2r_pre<P>(v, m, i, s_twf + zero);
-      }
-      pencil_fft<P, +1, 0, Lds, true>(v, i, s_twh + zero, lds);
-      // the output scale iscale (a power of two) is applied inside the
-      // barrier's FMA and to the sum of squares, where it is exact
-      // (the FP16-storage path evaluates it in FP32, far inside its own error)
-      const double A = rc[r0 + q].A, B = rc[r0 + q].B;
-      const BarrierF32 bk = barrier_f32(A, B, iscale);
-      const float R = rc[r0 + q].R;
-      float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const float z0 = kH ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
-        const float z1 = kH ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
-        acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
-        if constexpr (RB) {
-          if (z0 > best[e].x) {
-            best[e].x = z0;
-            bR0[e] = R;
-          }
-          if (z1 > best[e].y) {
-            best[e].y = z1;
-            bR1[e] = R;
-          }
-        } else {  // same zreion as the strict comparison: z is never NaN (a radius
-                  // with sigma(R) > sigma_min has B = -inf, see barrier_offset)
-          best[e].x = zmax_nonneg(best[e].x, z0);
-          best[e].y = zmax_nonneg(best[e].y, z1);
+        if (z1 > best[e].y) {
+          best[e].y = z1;
+          bR1[e] = R;
         }
+      } else {  // same zreion as the strict comparison: z is never NaN (a radius
+                // with sigma(R) > sigma_min has B = -inf, see barrier_offset)
+        best[e].x = zmax_nonneg(best[e].x, z0);
+        best[e].y = zmax_nonneg(best[e].y, z1);
       }
-      const double wsum = kH ? (double)wave_sum_f(valid ? acc : 0.0f) * sc2 : wave_sum(valid ? (double)acc * sc2 : 0.0);
-      if (lane == 0) red[w][q] += wsum;
     }
-    if (valid) {
-      float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
-      float2* bp = RB ? reinterpret_cast<float2*>(rbub + row * (2 * L)) : nullptr;
+    const double wsum = kH ? (double)wave_sum_f(valid ? acc : 0.0f) * sc2 : wave_sum(valid ? (double)acc * sc2 : 0.0);
+    if (lane == 0) red[w][q] += wsum;
+  }
+  if (valid) {
+    float2* zp = reinterpret_cast<float2*>(zre + row * (2 * L));
+    float2* bp = RB ? reinterpret_cast<float2*>(rbub + row * (2 * L)) : nullptr;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int n = i + e * T;
-        const float2 old = zp[n];
-        const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
-        if (u0 || u1) {
-          zp[n] = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
-          if (RB) {
-            const float2 ob = bp[n];
-            bp[n] = make_float2(u0 ? bR0[e] : ob.x, u1 ? bR1[e] : ob.y);
-          }
+    for (int e = 0; e < E; ++e) {
+      const int n = i + e * T;
+      const float2 old = zp[n];
+      const bool u0 = best[e].x > old.x, u1 = best[e].y > old.y;
+      if (u0 || u1) {
+        zp[n] = make_float2(u0 ? best[e].x : old.x, u1 ? best[e].y : old.y);
+        if (RB) {
+          const float2 ob = bp[n];
+          bp[n] = make_float2(u0 ? bR0[e] : ob.x, u1 ? bR1[e] : ob.y);
         }
       }
     }
   }
+}
+
+// The fused z pass (c2r + barrier + running max) over all rows: a
+// grid-stride kernel over row groups; one partial sum per block and radius.
+template <class P, int C, bool RB, class TS>
+__global__ void __launch_bounds__(256)
+    k_fz(const TS* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
+         const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
+         const float2* __restrict__ tw_half, const float2* __restrict__ tw_full) {
+  constexpr int ROWS = 256 / P::T;
+  __shared__ float2 smem[FzLds<P>::SMEM];
+  __shared__ double red[4][kFusedMaxRadii];
+  // twiddles in LDS (the blocks are persistent: copied once), the plan's
+  // stages transposed so that lanes with consecutive k read consecutive words;
+  // c2r glue: w^k for FP32 storage, the packed FP16 constants (P, Q) of
+  // c2r_glue_h2 for FP16 storage
+  __shared__ float2 s_twh[FzLds<P>::NTW];
+  __shared__ float2 s_twf[P::L];
+  fz_tables<P, TS>(red, s_twh, s_twf, tw_half, tw_full);
   __syncthreads();
+#pragma unroll 1
+  for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS)
+    fz_group<P, C, RB, TS>(Tbuf, zre, rbub, nrows, rc, r0, nb, iscale, base, s_twh, s_twf, smem, red);
+  __syncthreads();
+  const int t = threadIdx.x;
   if (t < nb) {
     double s = 0.0;
     for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k][t];
@@ -661,14 +739,14 @@ __global__ void k_twst(float2* out) {
 
 template <int N>
 static void twst_sizes(int* full, int* half) {
-  *full = plan_tw_size<Plan<N>>();
+  *full = plan_tw_size<TilePlan<N>>();
   *half = plan_tw_size<Plan<N / 2>>();
 }
 
 template <int N>
 static hipError_t twst_impl(float2* full, float2* half, hipStream_t s) {
-  constexpr int a = plan_tw_size<Plan<N>>(), b = plan_tw_size<Plan<N / 2>>();
-  if (a > 0) hipLaunchKernelGGL((k_twst<Plan<N>>), dim3((a + 255) / 256), dim3(256), 0, s, full);
+  constexpr int a = plan_tw_size<TilePlan<N>>(), b = plan_tw_size<Plan<N / 2>>();
+  if (a > 0) hipLaunchKernelGGL((k_twst<TilePlan<N>>), dim3((a + 255) / 256), dim3(256), 0, s, full);
   if (b > 0) hipLaunchKernelGGL((k_twst<Plan<N / 2>>), dim3((b + 255) / 256), dim3(256), 0, s, half);
   return hipGetLastError();
 }
@@ -757,7 +835,7 @@ static hipError_t inv_z_impl(const TS* in, float* out, float iscale, const float
 template <int N, int DIR, class TI, class TO>
 static hipError_t tile_impl(const TI* in, TO* out, const TileMap& mp, float oscale, const float2* tw,
                             hipStream_t s) {
-  using P = Plan<N>;
+  using P = TilePlan<N>;
   constexpr int C = Blk<N>::C;
   const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
   hipLaunchKernelGGL((k_tile<P, C, DIR, TI, TO>), dim3((unsigned)ntiles), dim3(P::T * C), 0, s, in, out, mp.in_sA,
@@ -777,7 +855,7 @@ static int resident_blocks(F f, int threads) {
 
 template <int N, int DIR, class TS>
 static hipError_t tile_ip_impl(TS* d, const TileMap& mp, float oscale, const float2* tw, hipStream_t s) {
-  using P = Plan<N>;
+  using P = TilePlan<N>;
   constexpr int C = Blk<N>::C;
   const long long ntiles = (long long)mp.nOuter * mp.nColBlocks;
   static const int resident = resident_blocks(k_tile_ip<P, C, DIR, TS>, P::T * C);
