@@ -272,6 +272,22 @@ int prc_d2z_autotune(prc_d2z_ctx* ctx, prc_d2z_tuning* out);
  * context's captured run is submitted on its own stream, then all are awaited. */
 int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n);
 
+/* Multi-box group: boxes of N = 64, 128, 192 or 256 (mode FUSED_FP32, or
+ * FUSED_MIXED, whose kernels are the FP32 ones at these N), the same Nscales
+ * and radii per batch, run together -- one persistent launch per pass walks a
+ * work list of (box, tile) items with each box's own plan, and the whole step
+ * is one captured graph.  The boxes keep their contexts (create, set_delta or
+ * generate_delta, get_zreion as usual); results equal each box's prc_d2z_run
+ * bit for bit (sigma_grid may differ in the last bits: its partial sums are
+ * grouped differently).  No counterpart in the reference, which runs one box
+ * per process. */
+typedef struct prc_d2z_group prc_d2z_group;
+int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out);
+int prc_d2z_group_run(prc_d2z_group* grp);
+void prc_d2z_group_destroy(prc_d2z_group* grp);
+/* Message of the last failed call on grp (grp == NULL: last failed create). */
+const char* prc_d2z_group_last_error(const prc_d2z_group* grp);
+
 /* Replay prc_d2z_run as one captured hipGraph (on by default; the graph is
  * captured on the first run of the context).  Timing mode runs eagerly. */
 int prc_d2z_set_graph(prc_d2z_ctx* ctx, int enable);

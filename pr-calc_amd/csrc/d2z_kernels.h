@@ -151,6 +151,44 @@ int fz_blocks(int N, bool half, bool rb);
 hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
                      int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
                      hipStream_t s);
+// ------------------------------------------------------ multi-box launches
+// One box of a persistent multi-box launch (the buffers of its context).
+// Boxes of N in {64, 128, 192, 256} run in one launch per pass: the blocks
+// walk a list of work items (box, tile or row group) and every item runs the
+// same device code as the single-box kernel of its N, so results are those of
+// the box's own run.
+struct MBox {
+  const float* delta;
+  float2* S;
+  float2* D0;
+  float2* DN;
+  float2* T;
+  float* zre;
+  float* rbub;  // nullptr: bubble radii not recorded
+  const RadiusConst* rc;
+  double* part;  // [Nscales][kFzMaxBlocks] per-block sums of the z pass
+  const float2* st_full;  // stage twiddles of the tile plan (x, y passes)
+  const float2* st_half;  // stage twiddles of the half-length z plan
+  const float2* tw_full;  // exp(2 pi i m / N), m < N
+  const float2* st_fx;    // stage twiddles of the fused x plan
+  const float4* wcoef;    // the box's W(x) table
+  float* sigma_grid;      // [Nscales] output of the sums of squares
+  int wM;
+  float winvh, dk, invN3, dinit;
+  int N, Nscales;
+};
+constexpr int kMBoxMax = 256;  // boxes per multi-box launch
+bool mbox_supported(int N);
+// passes of a multi-box step; `first` holds nbox + 1 item offsets per pass
+enum class MBPass { kFill, kFwdZ, kFwdY, kFwdX, kUnpack, kFx, kInvY, kFz, kSigma };
+long long mbox_items(MBPass pass, int N, int nb);
+int mbox_blocks(MBPass pass, bool rb);  // resident capacity of the pass's kernel (persistent grid)
+// r0, nb: the batch's radii (x pass: this launch's radii); toff: the x pass's
+// first intermediate plane within the batch; maxM: the largest W table of the
+// boxes (x pass LDS); zgrid: the z pass's grid, which strides its partial sums
+hipError_t launch_mbox(MBPass pass, const MBox* boxes, const long long* first, int nbox, long long nitems, int r0,
+                       int nb, int toff, int maxM, int nblk, int zgrid, bool rb, hipStream_t s);
+
 // element-wise and set-up kernels (d2z_kernels.hip)
 hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long total, const RadiusConst* rc, int r,
                           hipStream_t s);

@@ -708,6 +708,363 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ============================================= multi-box persistent kernels
+// One launch per pass covers every box of a group (N in {64, 128, 192, 256});
+// grid = the kernel's resident capacity, and each block walks the work list:
+// item t of round k is t = k G + (block's slot), the slot XCD-aware inside
+// full rounds (consecutive items on one XCD, as the single-box kernels'
+// tiles).  Box b owns items [first[b], first[b+1]).  Every item runs the body
+// of the single-box kernel for its N, on that box's buffers.
+template <class F>
+__device__ __forceinline__ void mb_switch(int N, F&& f) {
+  switch (N) {
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    case 192: f(std::integral_constant<int, 192>{}); break;
+    case 256: f(std::integral_constant<int, 256>{}); break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ int mbox_of(const long long* __restrict__ first, int nbox, long long t) {
+  int lo = 0, hi = nbox - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// item of round k for this block, or -1 past the end of the list
+__device__ __forceinline__ long long mbox_item(long long k, long long nitems) {
+  const long long G = gridDim.x, base = k * G;
+  if (base + G <= nitems) return base + xcd_tile(blockIdx.x, G);
+  return (base + blockIdx.x < nitems) ? base + blockIdx.x : -1;
+}
+
+template <int N>
+__device__ __forceinline__ TileMap mb_tile_map(MBPass pass, int nb) {
+  constexpr int C = Blk<N>::C;
+  constexpr long long NC = (long long)N * C;
+  TileMap m{};
+  m.nOuter = N;
+  m.nColBlocks = (N / 2) / C;
+  m.sCB = (long long)N * N * C;
+  if (pass == MBPass::kFwdX) {  // T layout (x, ky) -> S layout (ky, kx)
+    m.in_sA = NC;
+    m.in_sO = C;
+    m.out_sA = C;
+    m.out_sO = NC;
+  } else {  // y in place; the inverse y covers the batch's nb intermediates
+    m.in_sA = m.out_sA = C;
+    m.in_sO = m.out_sO = NC;
+    if (pass == MBPass::kInvY) m.nColBlocks *= nb;
+  }
+  return m;
+}
+
+template <int N>
+constexpr long long mb_items(MBPass pass, int nb) {
+  constexpr long long n2 = (long long)N * N;
+  switch (pass) {
+    case MBPass::kFill: return (n2 * N + 4095) / 4096;
+    case MBPass::kFwdZ: return n2 / (256 / Plan<N / 2>::T);
+    case MBPass::kFwdY:
+    case MBPass::kFwdX: return (long long)N * ((N / 2) / Blk<N>::C);
+    case MBPass::kInvY: return (long long)N * ((N / 2) / Blk<N>::C) * nb;
+    case MBPass::kUnpack: return (n2 + 255) / 256;
+    case MBPass::kFx: return (long long)N * ((N / 2) / FxCols<N>::C) * (FxCols<N>::C / FxCols<N>::CF);
+    case MBPass::kFz: return (n2 + (256 / Plan<N / 2>::T) - 1) / (256 / Plan<N / 2>::T);
+    default: return 0;
+  }
+}
+
+// LDS (bytes) of each pass's body, the largest over the supported N
+template <int N>
+constexpr size_t mb_lds_n(MBPass pass) {
+  using PZ = Plan<N / 2>;
+  using PF = typename FxCols<N>::P;
+  switch (pass) {
+    case MBPass::kFwdZ: return sizeof(float2) * zrow_lds<PZ>();
+    case MBPass::kFwdY:
+    case MBPass::kFwdX:
+    case MBPass::kInvY: return sizeof(float2) * TilePlan<N>::L * Blk<N>::C;
+    case MBPass::kFx:
+      return sizeof(float2) * FxLds<PF, FxCols<N>::CF>::SMEM + sizeof(float) * FxLds<PF, FxCols<N>::CF>::KTAB;
+    case MBPass::kFz:
+      return sizeof(float2) * (FzLds<PZ>::SMEM + FzLds<PZ>::NTW + PZ::L);
+    default: return 0;
+  }
+}
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t mb_lds(MBPass pass) {
+  return cmax(cmax(mb_lds_n<64>(pass), mb_lds_n<128>(pass)), cmax(mb_lds_n<192>(pass), mb_lds_n<256>(pass)));
+}
+// fixed part of the x pass's LDS (before the W table) and of the z pass's
+// twiddle region (after the rows)
+constexpr size_t kMbFzTwOffset = cmax(cmax(sizeof(float2) * FzLds<Plan<32>>::SMEM, sizeof(float2) * FzLds<Plan<64>>::SMEM),
+                                      cmax(sizeof(float2) * FzLds<Plan<96>>::SMEM, sizeof(float2) * FzLds<Plan<128>>::SMEM));
+constexpr int kMbFzNtw = (int)cmax(cmax(FzLds<Plan<32>>::NTW, FzLds<Plan<64>>::NTW),
+                                   cmax(FzLds<Plan<96>>::NTW, FzLds<Plan<128>>::NTW));
+
+__global__ void __launch_bounds__(256)
+    k_mb_fill(const MBox* __restrict__ boxes, const long long* __restrict__ first, int nbox, long long nitems) {
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const long long n = (long long)boxes[b].N * boxes[b].N * boxes[b].N;
+    const long long i0 = (t - first[b]) * 4096;
+    float* zre = boxes[b].zre;
+    float* rb = boxes[b].rbub;
+    for (long long i = i0 + threadIdx.x; i < i0 + 4096 && i < n; i += 256) {
+      zre[i] = 0.0f;
+      if (rb) rb[i] = 0.0f;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mb_fwdz(const MBox* __restrict__ boxes, const long long* __restrict__ first, int nbox, long long nitems) {
+  extern __shared__ float4 mb_lds4[];
+  float2* smem = reinterpret_cast<float2*>(mb_lds4);
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const MBox& B = boxes[b];
+    const long long item = t - first[b];
+    mb_switch(B.N, [&](auto nc) {
+      constexpr int N = decltype(nc)::value;
+      using P = Plan<N / 2>;
+      fwd_z_rows<P, Blk<N>::C>(B.delta, B.T, (long long)N * N, B.dinit, B.st_half, B.tw_full, item * (256 / P::T),
+                               smem);
+    });
+    __syncthreads();  // the LDS is reused by the next item
+  }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_mb_tile(const MBox* __restrict__ boxes, const long long* __restrict__ first,
+                                                 int nbox, long long nitems, MBPass pass, int nb) {
+  extern __shared__ float4 mb_lds4[];
+  float2* smem = reinterpret_cast<float2*>(mb_lds4);
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const MBox& B = boxes[b];
+    const long long item = t - first[b];
+    mb_switch(B.N, [&](auto nc) {
+      constexpr int N = decltype(nc)::value;
+      using P = TilePlan<N>;
+      static_assert(P::T * Blk<N>::C == 256, "multi-box tiles are 256-thread blocks");
+      const TileMap m = mb_tile_map<N>(pass, nb);
+      const float2* in = B.T;
+      float2* out = (pass == MBPass::kFwdX) ? B.S : B.T;
+      tile_body<P, Blk<N>::C, DIR, float2, float2>(in, out, m.in_sA, m.in_sO, m.out_sA, m.out_sO, m.sCB, m.nOuter,
+                                                   1.0f, B.st_full, item, smem);
+    });
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mb_unpack(const MBox* __restrict__ boxes, const long long* __restrict__ first, int nbox, long long nitems) {
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const MBox& B = boxes[b];
+    const int N = B.N;
+    const long long id = (t - first[b]) * 256 + threadIdx.x;
+    if (id >= (long long)N * N) continue;
+    const int kx = (int)(id / N), ky = (int)(id % N);
+    const int mx = (N - kx) % N, my = (N - ky) % N;
+    constexpr int C = 16;  // Blk<N>::C for every multi-box N
+    const float2 p = B.S[((long long)ky * N + kx) * C];
+    const float2 q = cconj(B.S[((long long)my * N + mx) * C]);
+    B.D0[id] = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y + q.y));
+    const float2 d = make_float2(0.5f * (p.x - q.x), 0.5f * (p.y - q.y));
+    B.DN[id] = make_float2(d.y, -d.x);
+  }
+}
+
+// the x pass: the box's W(x) table is loaded into LDS when the block moves to
+// a new box (block-uniform); the body's first barrier covers the load
+__global__ void __launch_bounds__(256) k_mb_fx(const MBox* __restrict__ boxes, const long long* __restrict__ first,
+                                               int nbox, long long nitems, int r0, int nb, int toff, int maxM) {
+  extern __shared__ float4 mb_lds4[];
+  float4* wsh = mb_lds4;
+  float2* smem = reinterpret_cast<float2*>(mb_lds4 + maxM);
+  int loaded = -1;
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const MBox& B = boxes[b];
+    if (b != loaded) {
+      __syncthreads();  // the previous box's table is no longer read
+      for (int q = threadIdx.x; q < B.wM; q += 256) wsh[q] = B.wcoef[q];
+      loaded = b;
+    }
+    const long long item = t - first[b];
+    mb_switch(B.N, [&](auto nc) {
+      constexpr int N = decltype(nc)::value;
+      using P = typename FxCols<N>::P;
+      constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
+      static_assert(P::T * CF == 256, "multi-box x tiles are 256-thread blocks");
+      float* ktab = reinterpret_cast<float*>(smem + FxLds<P, CF>::SMEM);
+      float2* T = B.T + (long long)toff * ((long long)N * N * (N / 2));
+      fx_body<P, C, CF, float2>(B.S, B.D0, B.DN, T, B.dk, B.invN3, B.rc, r0, nb, B.st_fx, wsh, B.wM, B.winvh, item,
+                                smem, ktab);
+    });
+    __syncthreads();
+  }
+}
+
+// the z pass: per-radius sums of squares of the box a block is on are kept in
+// LDS and written to that box's partials (slot = block) when the block moves
+// on; each block first zeroes its slot of every box, so boxes it never visits
+// read 0 there
+template <bool RB>
+__global__ void __launch_bounds__(256) k_mb_fz(const MBox* __restrict__ boxes, const long long* __restrict__ first,
+                                               int nbox, long long nitems, int r0, int nb) {
+  extern __shared__ float4 mb_lds4[];
+  float2* smem = reinterpret_cast<float2*>(mb_lds4);
+  float2* s_twh = reinterpret_cast<float2*>(reinterpret_cast<char*>(mb_lds4) + kMbFzTwOffset);
+  float2* s_twf = s_twh + kMbFzNtw;
+  __shared__ double red[4][kFusedMaxRadii];
+  const long long G = gridDim.x;
+  for (int q = threadIdx.x; q < nbox * nb; q += 256)
+    boxes[q / nb].part[(long long)(r0 + q % nb) * G + blockIdx.x] = 0.0;
+  auto flush = [&](int b) {
+    __syncthreads();
+    if ((int)threadIdx.x < nb) {
+      double s = 0.0;
+      for (int w = 0; w < 4; ++w) s += red[w][threadIdx.x];
+      boxes[b].part[(long long)(r0 + threadIdx.x) * G + blockIdx.x] = s;
+    }
+    __syncthreads();
+  };
+  int cur = -1;
+  for (long long k = 0;; ++k) {
+    const long long t = mbox_item(k, nitems);
+    if (t < 0) break;
+    const int b = mbox_of(first, nbox, t);
+    const MBox& B = boxes[b];
+    if (b != cur) {
+      if (cur >= 0) flush(cur);
+      else __syncthreads();  // the zeroing above precedes this block's first flush
+      mb_switch(B.N, [&](auto nc) {
+        constexpr int N = decltype(nc)::value;
+        fz_tables<Plan<N / 2>, float2>(red, s_twh, s_twf, B.st_half, B.tw_full);
+      });
+      __syncthreads();
+      cur = b;
+    }
+    const long long item = t - first[b];
+    mb_switch(B.N, [&](auto nc) {
+      constexpr int N = decltype(nc)::value;
+      using P = Plan<N / 2>;
+      fz_group<P, Blk<N>::C, RB, float2>(B.T, B.zre, RB ? B.rbub : nullptr, (long long)N * N, B.rc, r0, nb, 1.0f,
+                                         item * (256 / P::T), s_twh, s_twf, smem, red);
+    });
+  }
+  if (cur >= 0) flush(cur);
+}
+
+// sigma_grid of every box and radius from the z pass's partials (zgrid slots)
+__global__ void k_mb_sigma(const MBox* __restrict__ boxes, int nbox, int zgrid) {
+  const int b = blockIdx.x / kFusedMaxRadii / 8, r = blockIdx.x % (kFusedMaxRadii * 8);
+  if (b >= nbox || r >= boxes[b].Nscales) return;
+  const MBox& B = boxes[b];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < zgrid; k += 64) a += B.part[(long long)r * zgrid + k];
+  a = wave_sum(a);
+  if (threadIdx.x == 0) B.sigma_grid[r] = (float)sqrt(a / ((double)B.N * B.N * B.N));
+}
+
+bool mbox_supported(int N) { return N == 64 || N == 128 || N == 192 || N == 256; }
+
+long long mbox_items(MBPass pass, int N, int nb) {
+  switch (N) {
+    case 64: return mb_items<64>(pass, nb);
+    case 128: return mb_items<128>(pass, nb);
+    case 192: return mb_items<192>(pass, nb);
+    case 256: return mb_items<256>(pass, nb);
+    default: return 0;
+  }
+}
+
+template <class F>
+static int mb_resident(F f, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 256, lds);
+  int g = (e == hipSuccess && cus > 0 && per_cu > 0) ? cus * per_cu : 1024;
+  return g > kFzMaxBlocks ? kFzMaxBlocks : g;
+}
+
+int mbox_blocks(MBPass pass, bool rb) {
+  switch (pass) {
+    case MBPass::kFill: return mb_resident(k_mb_fill, 0);
+    case MBPass::kFwdZ: return mb_resident(k_mb_fwdz, mb_lds(pass));
+    case MBPass::kFwdY:
+    case MBPass::kFwdX: return mb_resident(k_mb_tile<-1>, mb_lds(pass));
+    case MBPass::kInvY: return mb_resident(k_mb_tile<+1>, mb_lds(pass));
+    case MBPass::kUnpack: return mb_resident(k_mb_unpack, 0);
+    case MBPass::kFx: return mb_resident(k_mb_fx, mb_lds(pass) + sizeof(float4) * kWTableMax);
+    case MBPass::kFz: return rb ? mb_resident(k_mb_fz<true>, kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128))
+                                : mb_resident(k_mb_fz<false>, kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128));
+    default: return 1;
+  }
+}
+
+hipError_t launch_mbox(MBPass pass, const MBox* boxes, const long long* first, int nbox, long long nitems, int r0,
+                       int nb, int toff, int maxM, int nblk, int zgrid, bool rb, hipStream_t s) {
+  if (nbox < 1 || nbox > kMBoxMax || nblk < 1 || nitems < 0) return hipErrorInvalidValue;
+  const dim3 g(nblk), b(256);
+  switch (pass) {
+    case MBPass::kFill:
+      hipLaunchKernelGGL(k_mb_fill, g, b, 0, s, boxes, first, nbox, nitems);
+      break;
+    case MBPass::kFwdZ:
+      hipLaunchKernelGGL(k_mb_fwdz, g, b, mb_lds(pass), s, boxes, first, nbox, nitems);
+      break;
+    case MBPass::kFwdY:
+    case MBPass::kFwdX:
+      hipLaunchKernelGGL(k_mb_tile<-1>, g, b, mb_lds(pass), s, boxes, first, nbox, nitems, pass, nb);
+      break;
+    case MBPass::kInvY:
+      hipLaunchKernelGGL(k_mb_tile<+1>, g, b, mb_lds(pass), s, boxes, first, nbox, nitems, pass, nb);
+      break;
+    case MBPass::kUnpack:
+      hipLaunchKernelGGL(k_mb_unpack, g, b, 0, s, boxes, first, nbox, nitems);
+      break;
+    case MBPass::kFx:
+      if (maxM < 0 || maxM > kWTableMax || nb < 1 || nb > kFxMaxRadii) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_mb_fx, g, b, mb_lds(pass) + sizeof(float4) * maxM, s, boxes, first, nbox, nitems, r0, nb,
+                         toff, maxM);
+      break;
+    case MBPass::kFz: {
+      if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
+      const size_t lds = kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128);
+      if (rb)
+        hipLaunchKernelGGL(k_mb_fz<true>, g, b, lds, s, boxes, first, nbox, nitems, r0, nb);
+      else
+        hipLaunchKernelGGL(k_mb_fz<false>, g, b, lds, s, boxes, first, nbox, nitems, r0, nb);
+      break;
+    }
+    case MBPass::kSigma:
+      hipLaunchKernelGGL(k_mb_sigma, dim3((unsigned)(nbox * kFusedMaxRadii * 8)), dim3(64), 0, s, boxes, nbox, zgrid);
+      break;
+  }
+  return hipGetLastError();
+}
+
 // ============================================================ launchers
 #define PRC_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(192) X(256) X(512) X(1024) X(2048)
 

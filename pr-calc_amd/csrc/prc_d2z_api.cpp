@@ -431,6 +431,52 @@ int copy_field_out(prc_d2z_ctx* c, const float* dsrc, float* out, int layout) {
 
 }  // namespace
 
+// ================================================= multi-box groups
+// Boxes of several sizes run together: one persistent launch per pass covers
+// every box (the work lists of the MBox kernels), and the whole step is one
+// captured graph on the group's stream.  The boxes' own contexts hold the
+// buffers; results land in them as after prc_d2z_run.
+struct prc_d2z_group {
+  std::vector<prc_d2z_ctx*> ctx;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  MBox* d_boxes = nullptr;
+  int NS = 0, nbatch = 1, maxM = 0, zgrid = 1;
+  bool rb = false;
+  struct Launch {
+    MBPass pass;
+    const long long* first;
+    long long nitems;
+    int r0, nb, toff, nblk;
+  };
+  std::vector<Launch> seq;
+  std::vector<void*> allocs;
+  hipGraphExec_t graph = nullptr;
+  std::string err;
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+namespace {
+thread_local std::string g_group_error;
+
+// item offsets of pass (with nb radii) over the boxes, on the device
+hipError_t group_first(prc_d2z_group* g, MBPass pass, int nb, const long long** out, long long* nitems) {
+  std::vector<long long> f(g->ctx.size() + 1, 0);
+  for (size_t b = 0; b < g->ctx.size(); ++b) f[b + 1] = f[b] + mbox_items(pass, g->ctx[b]->N, nb);
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(long long) * f.size());
+  if (e != hipSuccess) return e;
+  g->allocs.push_back(d);
+  e = hipMemcpy(d, f.data(), sizeof(long long) * f.size(), hipMemcpyHostToDevice);
+  *out = static_cast<const long long*>(d);
+  *nitems = f.back();
+  return e;
+}
+}  // namespace
+
 // ======================================== GetHistory without allocation
 // Device scratch of prc_d2z_history_noalloc: the integer counts of the
 // NHISTORY bins and the lowest / highest occupied bin (the same buffers the
@@ -1377,6 +1423,167 @@ int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n) {
   for (int k = 0; k < n; ++k) {
     int rc = prc_d2z_sync(ctxs[k]);
     if (rc) return rc;
+  }
+  return PRC_D2Z_OK;
+}
+
+void prc_d2z_group_destroy(prc_d2z_group* g) {
+  if (!g) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->graph) (void)hipGraphExecDestroy(g->graph);
+  for (void* q : g->allocs) (void)hipFree(q);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  (void)hipSetDevice(dev);
+  delete g;
+}
+
+const char* prc_d2z_group_last_error(const prc_d2z_group* g) { return g ? g->err.c_str() : g_group_error.c_str(); }
+
+int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out) {
+  g_group_error.clear();
+  if (!ctxs || !out || n < 1 || n > kMBoxMax) {
+    g_group_error = "need 1.." + std::to_string(kMBoxMax) + " contexts";
+    return PRC_D2Z_ERR_BAD_ARG;
+  }
+  *out = nullptr;
+  const prc_d2z_ctx* c0 = ctxs[0];
+  for (int k = 0; k < n; ++k) {
+    const prc_d2z_ctx* c = ctxs[k];
+    std::string why;
+    if (!c) why = "null context";
+    else if (c->p.device != c0->p.device) why = "contexts on different devices";
+    else if (c->p.mode != PRC_D2Z_FUSED_FP32 && !(c->p.mode == PRC_D2Z_FUSED_MIXED && !use_split(c->p)))
+      why = "group boxes run the fused FP32 kernels (mode FUSED_FP32, or FUSED_MIXED below N = 1024)";
+    else if (!mbox_supported(c->N)) why = "N = " + std::to_string(c->N) + " (group boxes: N = 64, 128, 192, 256)";
+    else if (c->p.Nscales != c0->p.Nscales || c->nbatch != c0->nbatch) why = "Nscales and radii per batch must agree";
+    else if (c->p.Nscales > 8 * kFusedMaxRadii) why = "Nscales above 256";
+    else if ((c->p.RecordBubbleSizes != 0) != (c0->p.RecordBubbleSizes != 0)) why = "RecordBubbleSizes must agree";
+    if (!why.empty()) {
+      g_group_error = "context " + std::to_string(k) + ": " + why;
+      return PRC_D2Z_ERR_BAD_ARG;
+    }
+  }
+  prc_d2z_group* g = new (std::nothrow) prc_d2z_group();
+  if (!g) {
+    g_group_error = "host allocation failed";
+    return PRC_D2Z_ERR_OOM;
+  }
+  g->ctx.assign(ctxs, ctxs + n);
+  g->device = c0->p.device;
+  g->NS = c0->p.Nscales;
+  g->nbatch = c0->nbatch;
+  g->rb = c0->p.RecordBubbleSizes != 0;
+  auto bail = [&](int code, const std::string& m) {
+    g_group_error = m;
+    prc_d2z_group_destroy(g);
+    return code;
+  };
+  if (hipSetDevice(g->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
+  std::vector<MBox> boxes(n);
+  for (int k = 0; k < n; ++k) {
+    prc_d2z_ctx* c = ctxs[k];
+    MBox& B = boxes[k];
+    B.delta = c->d_delta;
+    B.S = c->d_S;
+    B.D0 = c->d_D0;
+    B.DN = c->d_DN;
+    B.T = c->d_T;
+    B.zre = c->d_zre;
+    B.rbub = c->p.RecordBubbleSizes ? c->d_rb : nullptr;
+    B.rc = c->d_rc;
+    B.part = c->d_part;
+    B.st_full = c->d_st_full;
+    B.st_half = c->d_st_half;
+    B.tw_full = c->d_tw_full;
+    B.st_fx = c->d_st_fx;
+    B.wcoef = c->wt.coef;
+    B.sigma_grid = c->d_sigma_grid;
+    B.wM = c->wt.M;
+    B.winvh = c->wt.invh;
+    B.dk = host_dk(c);
+    B.invN3 = (float)(1.0 / ((double)c->N * c->N * c->N));
+    B.dinit = c->sc.DInit;
+    B.N = c->N;
+    B.Nscales = c->p.Nscales;
+    g->maxM = std::max(g->maxM, c->wt.M);
+  }
+  hipError_t e = hipMalloc(&g->d_boxes, sizeof(MBox) * n);
+  if (e == hipSuccess) {
+    g->allocs.push_back(g->d_boxes);
+    e = hipMemcpy(g->d_boxes, boxes.data(), sizeof(MBox) * n, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("group descriptors: ") + hipGetErrorString(e));
+  // the launch sequence of one step
+  g->zgrid = mbox_blocks(MBPass::kFz, g->rb);
+  auto add = [&](MBPass pass, int r0, int nb, int toff, int items_nb) -> hipError_t {
+    prc_d2z_group::Launch L{pass, nullptr, 0, r0, nb, toff, 1};
+    hipError_t ee = group_first(g, pass, items_nb, &L.first, &L.nitems);
+    if (ee != hipSuccess) return ee;
+    const int cap = (pass == MBPass::kFz) ? g->zgrid : mbox_blocks(pass, g->rb);
+    L.nblk = (int)std::max<long long>(1, std::min<long long>(cap, L.nitems));
+    if (pass == MBPass::kFz) L.nblk = g->zgrid;
+    g->seq.push_back(L);
+    return hipSuccess;
+  };
+  if ((e = add(MBPass::kFill, 0, 0, 0, 1)) == hipSuccess && (e = add(MBPass::kFwdZ, 0, 0, 0, 1)) == hipSuccess &&
+      (e = add(MBPass::kFwdY, 0, 0, 0, 1)) == hipSuccess && (e = add(MBPass::kFwdX, 0, 0, 0, 1)) == hipSuccess)
+    e = add(MBPass::kUnpack, 0, 0, 0, 1);
+  for (int r0 = 0; e == hipSuccess && r0 < g->NS; r0 += g->nbatch) {
+    const int nb = std::min(g->nbatch, g->NS - r0);
+    for (int off = 0; e == hipSuccess && off < nb; off += kFxMaxRadii)
+      e = add(MBPass::kFx, r0 + off, std::min(kFxMaxRadii, nb - off), off, 1);
+    if (e == hipSuccess) e = add(MBPass::kInvY, r0, nb, 0, nb);
+    if (e == hipSuccess) e = add(MBPass::kFz, r0, nb, 0, 1);
+  }
+  if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("group work lists: ") + hipGetErrorString(e));
+  *out = g;
+  return PRC_D2Z_OK;
+}
+
+int prc_d2z_group_run(prc_d2z_group* g) {
+  if (!g) return PRC_D2Z_ERR_BAD_ARG;
+  (void)hipSetDevice(g->device);
+  for (prc_d2z_ctx* c : g->ctx) {
+    if (!c->have_delta) return g->fail(PRC_D2Z_ERR_STATE, "a box has no density field");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return g->fail(PRC_D2Z_ERR_HIP, "box stream");
+  }
+  const int nbox = (int)g->ctx.size();
+  auto enqueue = [&]() -> hipError_t {
+    for (const auto& L : g->seq) {
+      hipError_t e = launch_mbox(L.pass, g->d_boxes, L.first, nbox, L.nitems, L.r0, L.nb, L.toff, g->maxM, L.nblk,
+                                 g->zgrid, g->rb, g->stream);
+      if (e != hipSuccess) return e;
+    }
+    return launch_mbox(MBPass::kSigma, g->d_boxes, nullptr, nbox, 0, 0, 0, 0, 0, 1, g->zgrid, g->rb, g->stream);
+  };
+  if (!g->graph) {
+    hipError_t e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return g->fail(PRC_D2Z_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t le = enqueue();
+    hipGraph_t gr = nullptr;
+    e = hipStreamEndCapture(g->stream, &gr);
+    if (le != hipSuccess || e != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return g->fail(PRC_D2Z_ERR_HIP, std::string("group launch: ") + hipGetErrorString(le != hipSuccess ? le : e));
+    }
+    e = hipGraphInstantiate(&g->graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      g->graph = nullptr;
+      return g->fail(PRC_D2Z_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+  }
+  hipError_t e = hipGraphLaunch(g->graph, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) return g->fail(PRC_D2Z_ERR_HIP, std::string("group run: ") + hipGetErrorString(e));
+  for (prc_d2z_ctx* c : g->ctx) {
+    c->have_spectrum = true;
+    c->have_result = true;
   }
   return PRC_D2Z_OK;
 }

@@ -87,6 +87,7 @@ EXPORTS = [
     "prc_d2z_bubble_mfp", "prc_d2z_corrected_history", "prc_d2z_correct_zreion",
     "prc_d2z_get_radius_diag", "prc_d2z_set_scalar_overrides", "prc_d2z_history_noalloc",
     "prc_d2z_set_plan", "prc_d2z_get_plan",
+    "prc_d2z_group_create", "prc_d2z_group_run", "prc_d2z_group_destroy", "prc_d2z_group_last_error",
 ]
 
 _lib = None
@@ -145,6 +146,10 @@ def load_library(path: str | os.PathLike | None = None):
         "prc_d2z_history_noalloc": (C.c_int, [vp, C.c_int64, up]),
         "prc_d2z_set_plan": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "prc_d2z_get_plan": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "prc_d2z_group_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+        "prc_d2z_group_run": (C.c_int, [vp]),
+        "prc_d2z_group_destroy": (None, [vp]),
+        "prc_d2z_group_last_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -429,3 +434,40 @@ def history_noalloc(d_zreion: int, ncells: int) -> np.ndarray:
     if rc:
         raise D2ZError(rc, lib.prc_d2z_last_error(None).decode())
     return out
+
+
+class Group:
+    """Several boxes (N = 64..256, fused FP32 kernels) run together: one
+    persistent launch per pass over all of them (prc_d2z_group_*).  The boxes
+    stay usable as D2Z objects; after run() each holds its result."""
+
+    def __init__(self, boxes):
+        self.lib = load_library()
+        self.boxes = list(boxes)
+        arr = (C.c_void_p * len(self.boxes))(*[b._ctx.value for b in self.boxes])
+        self._g = C.c_void_p()
+        rc = self.lib.prc_d2z_group_create(arr, len(self.boxes), C.byref(self._g))
+        if rc:
+            raise D2ZError(rc, self.lib.prc_d2z_group_last_error(None).decode())
+
+    def run(self):
+        rc = self.lib.prc_d2z_group_run(self._g)
+        if rc:
+            raise D2ZError(rc, self.lib.prc_d2z_group_last_error(self._g).decode())
+
+    def close(self):
+        if self._g:
+            self.lib.prc_d2z_group_destroy(self._g)
+            self._g = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
