@@ -1,6 +1,6 @@
 """The half path (FUSED_HALF: FP32 arithmetic, FP16 storage of the two
 inter-pass intermediates, FP16 matrix-core stages at N = 1024) against the
-FP64 oracle.  The compensated mixed path (FUSED_HALF) at 1024^3 is checked in
+FP64 oracle.  The compensated mixed path (FUSED_MIXED) at 1024^3 is checked in
 test_gpu_parity.py::test_c3_size_1024 to FP32 tolerances.
 
 Tolerances (stated for the half path): max |dzreion| <= 0.02, the fraction of
@@ -19,6 +19,23 @@ from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_HALF
 pytestmark = pytest.mark.gpu
 
 
+def _where_cells_differ(a, b, tol, what):
+    """Print where |a - b| > tol: the cell count, the (x, y) z rows and x planes
+    they fall in, and the largest differences with their cells, so that a
+    failure names the pass (a z row, a y-z plane or scattered cells)."""
+    bad = np.abs(a.astype(np.float64) - b) > tol
+    n = int(bad.sum())
+    if n == 0:
+        return
+    x, y, zz = np.nonzero(bad)
+    rows = np.unique(x * a.shape[1] + y)
+    print(f"{what}: {n} cells above {tol}; {rows.size} z rows, x planes {np.unique(x)[:16].tolist()}, "
+          f"y {np.unique(y)[:16].tolist()}, z {np.unique(zz)[:16].tolist()}")
+    d = np.abs(a.astype(np.float64) - b)[bad]
+    for k in np.argsort(d)[::-1][:8]:
+        print(f"  cell ({x[k]}, {y[k]}, {zz[k]}): {a[x[k], y[k], zz[k]]} vs {b[x[k], y[k], zz[k]]}")
+
+
 # 192 runs the radix-3 plans (z c2r of length 96); every size but 1024 runs the
 # VALU z pass with its packed FP16 glue
 @pytest.mark.parametrize("N", [32, 64, 128, 192, 256])
@@ -30,11 +47,16 @@ def test_half_zreion_vs_oracle(N):
         d.run()
         z = d.zreion()
         tab = d.radius_table()
+        d.run()  # a second run of the same graph gives the same bits
+        z_again = d.zreion()
     op = oracle_params(lp)
     sc = oracle.setup_scalars(op)
     zr, sg, _ = oracle.delta2zreion(field(N), op, sc)
     m = zreion_metrics(z, zr)
     print(m)
+    _where_cells_differ(z, zr, 0.01, "run 1 vs oracle")
+    _where_cells_differ(z_again, z, 0.0, "run 2 vs run 1")
+    assert np.array_equal(z_again, z)
     assert m["max_abs"] <= 0.02
     assert m["frac_gt_0.01"] <= 1e-3
     assert m["xhii_max"] <= 1e-3

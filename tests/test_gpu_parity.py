@@ -240,11 +240,22 @@ def test_field_generator_matches_oracle():
 @pytest.mark.slow
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32])
 def test_c2_size_512(mode):
-    """512^3 / 1000 Mpc/h (the C2 box) on two of its 64 radii: zreion over the
-    subset against the FP64 oracle, plus size-independent properties."""
+    """512^3 / 1000 Mpc/h (the C2 box): the smoothed fields of radii 0 and 40
+    against the FP64 oracle, and zreion over those two radii (every other radius
+    pinned out with a sigma override above sigma_min, so that its barrier never
+    crosses) against the oracle's zreion over the same two radii, at the FP32
+    tolerance; size-independent properties of the full 64-radius run."""
     N, box = 512, 1000.0
+    radii = (0, 40)
     from prc_d2z import D2ZParams
     lp = D2ZParams(N=N, BoxSize=box, Nscales=64, Rmax=80.0, zeta=40.0, M_min=1.4706e8, mode=mode)
+    op = oracle_params(lp)
+    sc = oracle.setup_scalars(op)
+    sig2 = np.concatenate([[sc.sigma_min], sc.sigma]).astype(np.float32)
+    keep = np.zeros(lp.Nscales + 1, bool)
+    keep[0] = True
+    keep[[r + 1 for r in radii]] = True
+    sig2[~keep] = np.float32(1e30)
     announce(f"512^3 mode={mode}")
     with D2Z(lp) as d:
         d.generate_delta(1)
@@ -252,13 +263,16 @@ def test_c2_size_512(mode):
         d.run()
         z = d.zreion()
         tab = d.radius_table()
-        fields = {r: d.smoothed_field(r) for r in (0, 40)}
+        fields = {r: d.smoothed_field(r) for r in radii}
+        d.set_scalar_overrides(sigma=sig2)
+        d.run()
+        z2 = d.zreion()
     assert np.all(z >= 0) and np.isfinite(z).all()
     assert np.all(np.diff(tab[:, 1]) < 0)
-    op = oracle_params(lp)
-    sc = oracle.setup_scalars(op)
+    assert np.all(z2 <= z + 1e-4)  # a subset of the radii (sigma of radii 0 and 40 within ulps of the run's)
     dhat = oracle.forward(delta, sc.DInit)
     kmag = oracle.kmag_grid(N, sc.box_mpc)
+    zr = np.zeros((N, N, N), np.float64)
     for r, g in fields.items():
         ref = oracle.smooth(dhat, kmag, sc.R[r])
         err = np.max(np.abs(g - ref)) / np.sqrt(np.mean(ref ** 2))
@@ -266,6 +280,12 @@ def test_c2_size_512(mode):
         assert err <= 5e-5
         # sigma_grid of the run equals the rms of the smoothed field
         assert tab[r, 2] == pytest.approx(np.sqrt(np.mean(ref ** 2)), rel=1e-5)
+        zc = oracle.barrier(ref, sc.A, sc.B[r]).astype(np.float32).astype(np.float64)
+        np.copyto(zr, zc, where=zc > zr)
+    m = zreion_metrics(z2, zr.astype(np.float32))
+    print(f"512^3 radii {radii} mode {mode} vs FP64 oracle: {m}")
+    assert np.mean(z2 > 0) > 0.01
+    assert m["max_abs"] <= TOL_Z and m["frac_gt_0.01"] == 0.0 and m["xhii_max"] <= 1e-3
 
 
 def test_c3_size_1024():
