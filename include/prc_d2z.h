@@ -274,9 +274,11 @@ int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n);
 
 /* Multi-box group: boxes of N = 64, 128, 192 or 256 (mode FUSED_FP32, or
  * FUSED_MIXED, whose kernels are the FP32 ones at these N), the same Nscales
- * and radii per batch, run together -- one persistent launch per pass walks a
- * work list of (box, tile) items with each box's own plan, and the whole step
- * is one captured graph.  The boxes keep their contexts (create, set_delta or
+ * and radii per batch, run together -- the boxes are spread over four chains
+ * (streams) balanced by cells; on a chain, the boxes of one size go through
+ * one launch per pass that walks a work list of (box, tile) items with each
+ * box's own plan; the chains run side by side and the whole step is one
+ * captured graph.  The boxes keep their contexts (create, set_delta or
  * generate_delta, get_zreion as usual); results equal each box's prc_d2z_run
  * bit for bit (sigma_grid may differ in the last bits: its partial sums are
  * grouped differently).  No counterpart in the reference, which runs one box

@@ -153,10 +153,12 @@ hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub
                      hipStream_t s);
 // ------------------------------------------------------ multi-box launches
 // One box of a persistent multi-box launch (the buffers of its context).
-// Boxes of N in {64, 128, 192, 256} run in one launch per pass: the blocks
-// walk a list of work items (box, tile or row group) and every item runs the
-// same device code as the single-box kernel of its N, so results are those of
-// the box's own run.
+// Boxes of N in {64, 128, 192, 256} run together: the blocks of a launch walk
+// a list of work items (box, tile or row group) and every item runs the same
+// device code as the single-box kernel of its N, so results are those of the
+// box's own run.  The light passes take all boxes in one launch; the fused x
+// and z passes take one launch per box size (a size class), so that each is
+// compiled for one N and keeps that N's registers and occupancy.
 struct MBox {
   const float* delta;
   float2* S;
@@ -176,18 +178,22 @@ struct MBox {
   int wM;
   float winvh, dk, invN3, dinit;
   int N, Nscales;
+  int zgrid;  // grid of the z pass of this box's size class: the stride of part
 };
 constexpr int kMBoxMax = 256;  // boxes per multi-box launch
 bool mbox_supported(int N);
 // passes of a multi-box step; `first` holds nbox + 1 item offsets per pass
 enum class MBPass { kFill, kFwdZ, kFwdY, kFwdX, kUnpack, kFx, kInvY, kFz, kSigma };
 long long mbox_items(MBPass pass, int N, int nb);
-int mbox_blocks(MBPass pass, bool rb);  // resident capacity of the pass's kernel (persistent grid)
-// r0, nb: the batch's radii (x pass: this launch's radii); toff: the x pass's
-// first intermediate plane within the batch; maxM: the largest W table of the
-// boxes (x pass LDS); zgrid: the z pass's grid, which strides its partial sums
+// resident capacity of the pass's kernel (persistent grid); N: the size class
+// of the x and z passes (ignored by the others)
+int mbox_blocks(MBPass pass, bool rb, int N);
+// boxes[0..nbox) with item offsets first[0..nbox]; r0, nb: the batch's radii
+// (x pass: this launch's radii); toff: the x pass's first intermediate plane
+// within the batch; maxM: the largest W table of the boxes (x pass LDS); N:
+// the size class of an x or z pass launch (every box of it has that N)
 hipError_t launch_mbox(MBPass pass, const MBox* boxes, const long long* first, int nbox, long long nitems, int r0,
-                       int nb, int toff, int maxM, int nblk, int zgrid, bool rb, hipStream_t s);
+                       int nb, int toff, int maxM, int nblk, int N, bool rb, hipStream_t s);
 
 // element-wise and set-up kernels (d2z_kernels.hip)
 hipError_t launch_barrier(const float* dR, float* zre, float* rbub, long long total, const RadiusConst* rc, int r,

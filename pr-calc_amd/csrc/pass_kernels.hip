@@ -726,6 +726,18 @@ __device__ __forceinline__ void mb_switch(int N, F&& f) {
   }
 }
 
+// the same dispatch for host code (launch shapes of the per-size kernels)
+template <class F>
+inline void mb_switch_host(int N, F&& f) {
+  switch (N) {
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    case 128: f(std::integral_constant<int, 128>{}); break;
+    case 192: f(std::integral_constant<int, 192>{}); break;
+    case 256: f(std::integral_constant<int, 256>{}); break;
+    default: break;
+  }
+}
+
 __device__ __forceinline__ int mbox_of(const long long* __restrict__ first, int nbox, long long t) {
   int lo = 0, hi = nbox - 1;
   while (lo < hi) {
@@ -801,13 +813,6 @@ constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t mb_lds(MBPass pass) {
   return cmax(cmax(mb_lds_n<64>(pass), mb_lds_n<128>(pass)), cmax(mb_lds_n<192>(pass), mb_lds_n<256>(pass)));
 }
-// fixed part of the x pass's LDS (before the W table) and of the z pass's
-// twiddle region (after the rows)
-constexpr size_t kMbFzTwOffset = cmax(cmax(sizeof(float2) * FzLds<Plan<32>>::SMEM, sizeof(float2) * FzLds<Plan<64>>::SMEM),
-                                      cmax(sizeof(float2) * FzLds<Plan<96>>::SMEM, sizeof(float2) * FzLds<Plan<128>>::SMEM));
-constexpr int kMbFzNtw = (int)cmax(cmax(FzLds<Plan<32>>::NTW, FzLds<Plan<64>>::NTW),
-                                   cmax(FzLds<Plan<96>>::NTW, FzLds<Plan<128>>::NTW));
-
 __global__ void __launch_bounds__(256)
     k_mb_fill(const MBox* __restrict__ boxes, const long long* __restrict__ first, int nbox, long long nitems) {
   for (long long k = 0;; ++k) {
@@ -891,13 +896,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// the x pass: the box's W(x) table is loaded into LDS when the block moves to
-// a new box (block-uniform); the body's first barrier covers the load
-__global__ void __launch_bounds__(256) k_mb_fx(const MBox* __restrict__ boxes, const long long* __restrict__ first,
+// the x pass of one size class: the box's W(x) table is loaded into LDS when
+// the block moves to a new box (block-uniform); the body's first barrier
+// covers the load
+template <int N>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_mb_fx(const MBox* __restrict__ boxes, const long long* __restrict__ first,
                                                int nbox, long long nitems, int r0, int nb, int toff, int maxM) {
+  using P = typename FxCols<N>::P;
+  constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
+  static_assert(P::T * CF == 256, "multi-box x tiles are 256-thread blocks");
   extern __shared__ float4 mb_lds4[];
   float4* wsh = mb_lds4;
   float2* smem = reinterpret_cast<float2*>(mb_lds4 + maxM);
+  float* ktab = reinterpret_cast<float*>(smem + FxLds<P, CF>::SMEM);
   int loaded = -1;
   for (long long k = 0;; ++k) {
     const long long t = mbox_item(k, nitems);
@@ -909,32 +921,24 @@ __global__ void __launch_bounds__(256) k_mb_fx(const MBox* __restrict__ boxes, c
       for (int q = threadIdx.x; q < B.wM; q += 256) wsh[q] = B.wcoef[q];
       loaded = b;
     }
-    const long long item = t - first[b];
-    mb_switch(B.N, [&](auto nc) {
-      constexpr int N = decltype(nc)::value;
-      using P = typename FxCols<N>::P;
-      constexpr int C = FxCols<N>::C, CF = FxCols<N>::CF;
-      static_assert(P::T * CF == 256, "multi-box x tiles are 256-thread blocks");
-      float* ktab = reinterpret_cast<float*>(smem + FxLds<P, CF>::SMEM);
-      float2* T = B.T + (long long)toff * ((long long)N * N * (N / 2));
-      fx_body<P, C, CF, float2>(B.S, B.D0, B.DN, T, B.dk, B.invN3, B.rc, r0, nb, B.st_fx, wsh, B.wM, B.winvh, item,
-                                smem, ktab);
-    });
+    float2* T = B.T + (long long)toff * ((long long)N * N * (N / 2));
+    fx_body<P, C, CF, float2>(B.S, B.D0, B.DN, T, B.dk, B.invN3, B.rc, r0, nb, B.st_fx, wsh, B.wM, B.winvh,
+                              t - first[b], smem, ktab);
     __syncthreads();
   }
 }
 
-// the z pass: per-radius sums of squares of the box a block is on are kept in
-// LDS and written to that box's partials (slot = block) when the block moves
-// on; each block first zeroes its slot of every box, so boxes it never visits
-// read 0 there
-template <bool RB>
+// the z pass of one size class: per-radius sums of squares of the box a block
+// is on are kept in LDS and written to that box's partials (slot = block) when
+// the block moves on; each block first zeroes its slot of every box, so boxes
+// it never visits read 0 there
+template <int N, bool RB>
 __global__ void __launch_bounds__(256) k_mb_fz(const MBox* __restrict__ boxes, const long long* __restrict__ first,
                                                int nbox, long long nitems, int r0, int nb) {
-  extern __shared__ float4 mb_lds4[];
-  float2* smem = reinterpret_cast<float2*>(mb_lds4);
-  float2* s_twh = reinterpret_cast<float2*>(reinterpret_cast<char*>(mb_lds4) + kMbFzTwOffset);
-  float2* s_twf = s_twh + kMbFzNtw;
+  using P = Plan<N / 2>;
+  __shared__ float2 smem[FzLds<P>::SMEM];
+  __shared__ float2 s_twh[FzLds<P>::NTW];
+  __shared__ float2 s_twf[P::L];
   __shared__ double red[4][kFusedMaxRadii];
   const long long G = gridDim.x;
   for (int q = threadIdx.x; q < nbox * nb; q += 256)
@@ -957,31 +961,24 @@ __global__ void __launch_bounds__(256) k_mb_fz(const MBox* __restrict__ boxes, c
     if (b != cur) {
       if (cur >= 0) flush(cur);
       else __syncthreads();  // the zeroing above precedes this block's first flush
-      mb_switch(B.N, [&](auto nc) {
-        constexpr int N = decltype(nc)::value;
-        fz_tables<Plan<N / 2>, float2>(red, s_twh, s_twf, B.st_half, B.tw_full);
-      });
+      fz_tables<P, float2>(red, s_twh, s_twf, B.st_half, B.tw_full);
       __syncthreads();
       cur = b;
     }
-    const long long item = t - first[b];
-    mb_switch(B.N, [&](auto nc) {
-      constexpr int N = decltype(nc)::value;
-      using P = Plan<N / 2>;
-      fz_group<P, Blk<N>::C, RB, float2>(B.T, B.zre, RB ? B.rbub : nullptr, (long long)N * N, B.rc, r0, nb, 1.0f,
-                                         item * (256 / P::T), s_twh, s_twf, smem, red);
-    });
+    fz_group<P, Blk<N>::C, RB, float2>(B.T, B.zre, RB ? B.rbub : nullptr, (long long)N * N, B.rc, r0, nb, 1.0f,
+                                       (t - first[b]) * (256 / P::T), s_twh, s_twf, smem, red);
   }
   if (cur >= 0) flush(cur);
 }
 
-// sigma_grid of every box and radius from the z pass's partials (zgrid slots)
-__global__ void k_mb_sigma(const MBox* __restrict__ boxes, int nbox, int zgrid) {
+// sigma_grid of every box and radius from the z pass's partials (the box's
+// class grid of slots)
+__global__ void k_mb_sigma(const MBox* __restrict__ boxes, int nbox) {
   const int b = blockIdx.x / kFusedMaxRadii / 8, r = blockIdx.x % (kFusedMaxRadii * 8);
   if (b >= nbox || r >= boxes[b].Nscales) return;
   const MBox& B = boxes[b];
   double a = 0.0;
-  for (int k = threadIdx.x; k < zgrid; k += 64) a += B.part[(long long)r * zgrid + k];
+  for (int k = threadIdx.x; k < B.zgrid; k += 64) a += B.part[(long long)r * B.zgrid + k];
   a = wave_sum(a);
   if (threadIdx.x == 0) B.sigma_grid[r] = (float)sqrt(a / ((double)B.N * B.N * B.N));
 }
@@ -1008,7 +1005,14 @@ static int mb_resident(F f, size_t lds) {
   return g > kFzMaxBlocks ? kFzMaxBlocks : g;
 }
 
-int mbox_blocks(MBPass pass, bool rb) {
+// resident capacity of the x and z kernels of size class N
+template <int N>
+static int mb_blocks_n(MBPass pass, bool rb) {
+  if (pass == MBPass::kFx) return mb_resident(k_mb_fx<N>, mb_lds_n<N>(pass) + sizeof(float4) * kWTableMax);
+  return rb ? mb_resident(k_mb_fz<N, true>, 0) : mb_resident(k_mb_fz<N, false>, 0);
+}
+
+int mbox_blocks(MBPass pass, bool rb, int N) {
   switch (pass) {
     case MBPass::kFill: return mb_resident(k_mb_fill, 0);
     case MBPass::kFwdZ: return mb_resident(k_mb_fwdz, mb_lds(pass));
@@ -1016,15 +1020,31 @@ int mbox_blocks(MBPass pass, bool rb) {
     case MBPass::kFwdX: return mb_resident(k_mb_tile<-1>, mb_lds(pass));
     case MBPass::kInvY: return mb_resident(k_mb_tile<+1>, mb_lds(pass));
     case MBPass::kUnpack: return mb_resident(k_mb_unpack, 0);
-    case MBPass::kFx: return mb_resident(k_mb_fx, mb_lds(pass) + sizeof(float4) * kWTableMax);
-    case MBPass::kFz: return rb ? mb_resident(k_mb_fz<true>, kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128))
-                                : mb_resident(k_mb_fz<false>, kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128));
+    case MBPass::kFx:
+    case MBPass::kFz: {
+      int g = 1;
+      mb_switch_host(N, [&](auto nc) { g = mb_blocks_n<decltype(nc)::value>(pass, rb); });
+      return g;
+    }
     default: return 1;
   }
 }
 
+template <int N>
+static void launch_mb_class(MBPass pass, const MBox* boxes, const long long* first, int nbox, long long nitems, int r0,
+                            int nb, int toff, int maxM, dim3 g, bool rb, hipStream_t s) {
+  const dim3 b(256);
+  if (pass == MBPass::kFx)
+    hipLaunchKernelGGL(k_mb_fx<N>, g, b, mb_lds_n<N>(pass) + sizeof(float4) * maxM, s, boxes, first, nbox, nitems,
+                       r0, nb, toff, maxM);
+  else if (rb)
+    hipLaunchKernelGGL((k_mb_fz<N, true>), g, b, 0, s, boxes, first, nbox, nitems, r0, nb);
+  else
+    hipLaunchKernelGGL((k_mb_fz<N, false>), g, b, 0, s, boxes, first, nbox, nitems, r0, nb);
+}
+
 hipError_t launch_mbox(MBPass pass, const MBox* boxes, const long long* first, int nbox, long long nitems, int r0,
-                       int nb, int toff, int maxM, int nblk, int zgrid, bool rb, hipStream_t s) {
+                       int nb, int toff, int maxM, int nblk, int N, bool rb, hipStream_t s) {
   if (nbox < 1 || nbox > kMBoxMax || nblk < 1 || nitems < 0) return hipErrorInvalidValue;
   const dim3 g(nblk), b(256);
   switch (pass) {
@@ -1045,21 +1065,18 @@ hipError_t launch_mbox(MBPass pass, const MBox* boxes, const long long* first, i
       hipLaunchKernelGGL(k_mb_unpack, g, b, 0, s, boxes, first, nbox, nitems);
       break;
     case MBPass::kFx:
-      if (maxM < 0 || maxM > kWTableMax || nb < 1 || nb > kFxMaxRadii) return hipErrorInvalidValue;
-      hipLaunchKernelGGL(k_mb_fx, g, b, mb_lds(pass) + sizeof(float4) * maxM, s, boxes, first, nbox, nitems, r0, nb,
-                         toff, maxM);
-      break;
     case MBPass::kFz: {
-      if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
-      const size_t lds = kMbFzTwOffset + sizeof(float2) * (kMbFzNtw + 128);
-      if (rb)
-        hipLaunchKernelGGL(k_mb_fz<true>, g, b, lds, s, boxes, first, nbox, nitems, r0, nb);
-      else
-        hipLaunchKernelGGL(k_mb_fz<false>, g, b, lds, s, boxes, first, nbox, nitems, r0, nb);
+      if (!mbox_supported(N)) return hipErrorInvalidValue;
+      if (pass == MBPass::kFx && (maxM < 0 || maxM > kWTableMax || nb < 1 || nb > kFxMaxRadii))
+        return hipErrorInvalidValue;
+      if (pass == MBPass::kFz && (nb < 1 || nb > kFusedMaxRadii)) return hipErrorInvalidValue;
+      mb_switch_host(N, [&](auto nc) {
+        launch_mb_class<decltype(nc)::value>(pass, boxes, first, nbox, nitems, r0, nb, toff, maxM, g, rb, s);
+      });
       break;
     }
     case MBPass::kSigma:
-      hipLaunchKernelGGL(k_mb_sigma, dim3((unsigned)(nbox * kFusedMaxRadii * 8)), dim3(64), 0, s, boxes, nbox, zgrid);
+      hipLaunchKernelGGL(k_mb_sigma, dim3((unsigned)(nbox * kFusedMaxRadii * 8)), dim3(64), 0, s, boxes, nbox);
       break;
   }
   return hipGetLastError();

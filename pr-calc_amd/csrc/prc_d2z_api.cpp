@@ -115,7 +115,7 @@ struct prc_d2z_ctx {
   void* d_mats = nullptr;       // operand matrices of the matrix-core passes
   WTable wt{nullptr, 0, 0.0f};
   double2* d_mom = nullptr;   // plain: [Nscales][kMomentBlocks]
-  double* d_part = nullptr;   // fused: [Nscales][nblk_fz]
+  double* d_part = nullptr;   // fused: [Nscales][kFzMaxBlocks], strided by the z pass grid
   int nblk_fz = 0;
   // FFT plans of the fused x and z passes (PRC_D2Z_PLAN_*) and radii per x launch
   int xplan = PRC_D2Z_PLAN_VECTOR, zplan = PRC_D2Z_PLAN_VECTOR;
@@ -440,16 +440,20 @@ struct prc_d2z_group {
   std::vector<prc_d2z_ctx*> ctx;
   int device = 0;
   hipStream_t stream = nullptr;
-  MBox* d_boxes = nullptr;
-  int NS = 0, nbatch = 1, maxM = 0, zgrid = 1;
+  MBox* d_boxes = nullptr;  // sorted by N: each size class is a contiguous range
+  int NS = 0, nbatch = 1;
   bool rb = false;
   struct Launch {
     MBPass pass;
     const long long* first;
     long long nitems;
     int r0, nb, toff, nblk;
+    int b0, nbox, N, maxM;  // the boxes [b0, b0 + nbox) of the launch; N, maxM: x and z passes
+    int cls;                // chain: the stream of the launch
   };
   std::vector<Launch> seq;
+  std::vector<hipStream_t> cstreams;  // one per chain, forked from `stream` in the step graph
+  std::vector<hipEvent_t> events;     // the fork, then one join per class
   std::vector<void*> allocs;
   hipGraphExec_t graph = nullptr;
   std::string err;
@@ -462,10 +466,17 @@ struct prc_d2z_group {
 namespace {
 thread_local std::string g_group_error;
 
-// item offsets of pass (with nb radii) over the boxes, on the device
-hipError_t group_first(prc_d2z_group* g, MBPass pass, int nb, const long long** out, long long* nitems) {
-  std::vector<long long> f(g->ctx.size() + 1, 0);
-  for (size_t b = 0; b < g->ctx.size(); ++b) f[b + 1] = f[b] + mbox_items(pass, g->ctx[b]->N, nb);
+// grid of a group launch other than the z pass: one block per work item (the
+// blocks of the chains' concurrent launches interleave on the CUs)
+constexpr long long kGroupMaxBlocks = 1 << 20;
+// chains (streams) of a group step: the hardware queues of a process
+constexpr int kGroupChains = 4;
+
+// item offsets of pass (with nb radii) over the boxes [b0, b1), on the device
+hipError_t group_first(prc_d2z_group* g, MBPass pass, int nb, int b0, int b1, const long long** out,
+                       long long* nitems) {
+  std::vector<long long> f(b1 - b0 + 1, 0);
+  for (int b = b0; b < b1; ++b) f[b - b0 + 1] = f[b - b0] + mbox_items(pass, g->ctx[b]->N, nb);
   void* d = nullptr;
   hipError_t e = hipMalloc(&d, sizeof(long long) * f.size());
   if (e != hipSuccess) return e;
@@ -1435,6 +1446,8 @@ void prc_d2z_group_destroy(prc_d2z_group* g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->graph) (void)hipGraphExecDestroy(g->graph);
   for (void* q : g->allocs) (void)hipFree(q);
+  for (hipEvent_t ev : g->events) (void)hipEventDestroy(ev);
+  for (hipStream_t cs : g->cstreams) (void)hipStreamDestroy(cs);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   (void)hipSetDevice(dev);
   delete g;
@@ -1472,6 +1485,47 @@ int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out) {
     return PRC_D2Z_ERR_OOM;
   }
   g->ctx.assign(ctxs, ctxs + n);
+  // The boxes are spread over kGroupChains chains (streams) by cells, largest
+  // box first onto the least loaded chain; within a chain the boxes of one N
+  // form a segment, which runs the whole pass sequence with the kernels
+  // compiled for that N.  The chains run side by side, so the latency-bound x
+  // pass of one overlaps the copy-bound y pass of another.  Boxes are ordered
+  // by (chain, N): every segment is a contiguous range of the descriptors.
+  const int nchain = std::min(n, kGroupChains);
+  std::vector<int> chain_of(n, 0);
+  {
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ctxs[a]->ncell > ctxs[b]->ncell; });
+    std::vector<double> load(nchain, 0.0);
+    for (int k : order) {
+      const int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      chain_of[k] = c;
+      load[c] += (double)ctxs[k]->ncell;
+    }
+    std::vector<int> idx(n);
+    for (int k = 0; k < n; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      return chain_of[a] != chain_of[b] ? chain_of[a] < chain_of[b] : ctxs[a]->N < ctxs[b]->N;
+    });
+    std::vector<int> ch(n);
+    for (int k = 0; k < n; ++k) {
+      g->ctx[k] = ctxs[idx[k]];
+      ch[k] = chain_of[idx[k]];
+    }
+    chain_of = ch;
+  }
+  struct Class {  // a segment: boxes [b0, b1) of one N on one chain
+    int b0, b1, N, maxM, zgrid, chain;
+  };
+  std::vector<Class> classes;
+  for (int k = 0; k < n; ++k) {
+    if (classes.empty() || classes.back().N != g->ctx[k]->N || classes.back().chain != chain_of[k])
+      classes.push_back(Class{k, k, g->ctx[k]->N, 0, 1, chain_of[k]});
+    classes.back().b1 = k + 1;
+    classes.back().maxM = std::max(classes.back().maxM, g->ctx[k]->wt.M);
+  }
   g->device = c0->p.device;
   g->NS = c0->p.Nscales;
   g->nbatch = c0->nbatch;
@@ -1484,9 +1538,22 @@ int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out) {
   if (hipSetDevice(g->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
+  for (int k = 0; k < nchain + 1; ++k) {
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+      return bail(PRC_D2Z_ERR_HIP, "event creation failed");
+    g->events.push_back(ev);
+  }
+  for (int k = 0; k < nchain; ++k) {
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+      return bail(PRC_D2Z_ERR_HIP, "stream creation failed");
+    g->cstreams.push_back(cs);
+  }
+  for (Class& cl : classes) cl.zgrid = mbox_blocks(MBPass::kFz, g->rb, cl.N);
   std::vector<MBox> boxes(n);
   for (int k = 0; k < n; ++k) {
-    prc_d2z_ctx* c = ctxs[k];
+    prc_d2z_ctx* c = g->ctx[k];
     MBox& B = boxes[k];
     B.delta = c->d_delta;
     B.S = c->d_S;
@@ -1510,7 +1577,8 @@ int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out) {
     B.dinit = c->sc.DInit;
     B.N = c->N;
     B.Nscales = c->p.Nscales;
-    g->maxM = std::max(g->maxM, c->wt.M);
+    for (const Class& cl : classes)
+      if (cl.N == c->N) B.zgrid = cl.zgrid;
   }
   hipError_t e = hipMalloc(&g->d_boxes, sizeof(MBox) * n);
   if (e == hipSuccess) {
@@ -1518,27 +1586,28 @@ int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out) {
     e = hipMemcpy(g->d_boxes, boxes.data(), sizeof(MBox) * n, hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("group descriptors: ") + hipGetErrorString(e));
-  // the launch sequence of one step
-  g->zgrid = mbox_blocks(MBPass::kFz, g->rb);
-  auto add = [&](MBPass pass, int r0, int nb, int toff, int items_nb) -> hipError_t {
-    prc_d2z_group::Launch L{pass, nullptr, 0, r0, nb, toff, 1};
-    hipError_t ee = group_first(g, pass, items_nb, &L.first, &L.nitems);
+  // the launch sequence of one step, per segment, on its chain's stream
+  auto add = [&](int c, MBPass pass, int r0, int nb, int toff, int items_nb) -> hipError_t {
+    const Class& cl = classes[c];
+    prc_d2z_group::Launch L{pass, nullptr, 0, r0, nb, toff, 1, cl.b0, cl.b1 - cl.b0, cl.N, cl.maxM, cl.chain};
+    hipError_t ee = group_first(g, pass, items_nb, cl.b0, cl.b1, &L.first, &L.nitems);
     if (ee != hipSuccess) return ee;
-    const int cap = (pass == MBPass::kFz) ? g->zgrid : mbox_blocks(pass, g->rb);
-    L.nblk = (int)std::max<long long>(1, std::min<long long>(cap, L.nitems));
-    if (pass == MBPass::kFz) L.nblk = g->zgrid;
+    L.nblk = (int)std::max<long long>(1, std::min<long long>(kGroupMaxBlocks, L.nitems));
+    if (pass == MBPass::kFz) L.nblk = cl.zgrid;  // the stride of the boxes' partial sums
     g->seq.push_back(L);
     return hipSuccess;
   };
-  if ((e = add(MBPass::kFill, 0, 0, 0, 1)) == hipSuccess && (e = add(MBPass::kFwdZ, 0, 0, 0, 1)) == hipSuccess &&
-      (e = add(MBPass::kFwdY, 0, 0, 0, 1)) == hipSuccess && (e = add(MBPass::kFwdX, 0, 0, 0, 1)) == hipSuccess)
-    e = add(MBPass::kUnpack, 0, 0, 0, 1);
-  for (int r0 = 0; e == hipSuccess && r0 < g->NS; r0 += g->nbatch) {
-    const int nb = std::min(g->nbatch, g->NS - r0);
-    for (int off = 0; e == hipSuccess && off < nb; off += kFxMaxRadii)
-      e = add(MBPass::kFx, r0 + off, std::min(kFxMaxRadii, nb - off), off, 1);
-    if (e == hipSuccess) e = add(MBPass::kInvY, r0, nb, 0, nb);
-    if (e == hipSuccess) e = add(MBPass::kFz, r0, nb, 0, 1);
+  for (int c = 0; e == hipSuccess && c < (int)classes.size(); ++c) {
+    if ((e = add(c, MBPass::kFill, 0, 0, 0, 1)) == hipSuccess && (e = add(c, MBPass::kFwdZ, 0, 0, 0, 1)) == hipSuccess &&
+        (e = add(c, MBPass::kFwdY, 0, 0, 0, 1)) == hipSuccess && (e = add(c, MBPass::kFwdX, 0, 0, 0, 1)) == hipSuccess)
+      e = add(c, MBPass::kUnpack, 0, 0, 0, 1);
+    for (int r0 = 0; e == hipSuccess && r0 < g->NS; r0 += g->nbatch) {
+      const int nb = std::min(g->nbatch, g->NS - r0);
+      for (int off = 0; e == hipSuccess && off < nb; off += kFxMaxRadii)
+        e = add(c, MBPass::kFx, r0 + off, std::min(kFxMaxRadii, nb - off), off, 1);
+      if (e == hipSuccess) e = add(c, MBPass::kInvY, r0, nb, 0, nb);
+      if (e == hipSuccess) e = add(c, MBPass::kFz, r0, nb, 0, 1);
+    }
   }
   if (e != hipSuccess) return bail(PRC_D2Z_ERR_OOM, std::string("group work lists: ") + hipGetErrorString(e));
   *out = g;
@@ -1553,13 +1622,23 @@ int prc_d2z_group_run(prc_d2z_group* g) {
     if (hipStreamSynchronize(c->stream) != hipSuccess) return g->fail(PRC_D2Z_ERR_HIP, "box stream");
   }
   const int nbox = (int)g->ctx.size();
+  // fork the class streams off the group's stream, run each class's chain,
+  // join, then the sigma_grid reduction of every box
   auto enqueue = [&]() -> hipError_t {
-    for (const auto& L : g->seq) {
-      hipError_t e = launch_mbox(L.pass, g->d_boxes, L.first, nbox, L.nitems, L.r0, L.nb, L.toff, g->maxM, L.nblk,
-                                 g->zgrid, g->rb, g->stream);
-      if (e != hipSuccess) return e;
+    const int nc = (int)g->cstreams.size();
+    hipError_t e = hipEventRecord(g->events[0], g->stream);
+    for (int c = 0; e == hipSuccess && c < nc; ++c) e = hipStreamWaitEvent(g->cstreams[c], g->events[0], 0);
+    for (size_t k = 0; e == hipSuccess && k < g->seq.size(); ++k) {
+      const auto& L = g->seq[k];
+      e = launch_mbox(L.pass, g->d_boxes + L.b0, L.first, L.nbox, L.nitems, L.r0, L.nb, L.toff, L.maxM, L.nblk, L.N,
+                      g->rb, g->cstreams[L.cls]);
     }
-    return launch_mbox(MBPass::kSigma, g->d_boxes, nullptr, nbox, 0, 0, 0, 0, 0, 1, g->zgrid, g->rb, g->stream);
+    for (int c = 0; e == hipSuccess && c < nc; ++c) {
+      e = hipEventRecord(g->events[c + 1], g->cstreams[c]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, g->events[c + 1], 0);
+    }
+    if (e != hipSuccess) return e;
+    return launch_mbox(MBPass::kSigma, g->d_boxes, nullptr, nbox, 0, 0, 0, 0, 0, 1, 0, g->rb, g->stream);
   };
   if (!g->graph) {
     hipError_t e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);

@@ -437,9 +437,10 @@ def history_noalloc(d_zreion: int, ncells: int) -> np.ndarray:
 
 
 class Group:
-    """Several boxes (N = 64..256, fused FP32 kernels) run together: one
-    persistent launch per pass over all of them (prc_d2z_group_*).  The boxes
-    stay usable as D2Z objects; after run() each holds its result."""
+    """Several boxes (N = 64..256, fused FP32 kernels) run together
+    (prc_d2z_group_*): four chains balanced by cells, one launch per pass for
+    the boxes of one size on a chain, one graph per step.  The boxes stay
+    usable as D2Z objects; after run() each holds its result."""
 
     def __init__(self, boxes):
         self.lib = load_library()

@@ -1,5 +1,6 @@
-"""Multi-box groups: boxes of several sizes in one persistent launch per pass
-(prc_d2z_group_*), through the C ABI.
+"""Multi-box groups: boxes of several sizes run together, one launch per pass
+for the boxes of one size on each of four chains (prc_d2z_group_*), through
+the C ABI.
 
 * The uneven-box workload: 16 x 128^3 (200 Mpc/h), 16 x 192^3 (300 Mpc/h) and
   16 x 256^3 (400 Mpc/h), distinct seeds, 48 radii each (one cell to 50
