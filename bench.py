@@ -221,32 +221,42 @@ def main():
 
     kernels = kt["kernels"]
     dom = max(kernels.items(), key=lambda kv: kv[1][0])
-    dom_name, (dom_ms, dom_n) = dom
+    dom_name = dom[0]
     # radii per fused launch as the library runs them (it may lower the
     # request to fit the intermediates in HBM)
     nb = d.launch_config()["radii_per_launch"] if args.mode != "plain" else 1
     plan = d.plan() if args.mode != "plain" else None
     fx_nb = plan["x_radii_per_launch"] if plan else FX_MAX_RADII
     store = STORE_BYTES[args.mode]
-    # algorithmic bytes of the dominant kernel over the step (the last fused
-    # batch may carry fewer radii; the x pass fills a batch in launches of at
-    # most FX_MAX_RADII, each reading the spectrum), per launch on average
-    if args.mode == "plain" or dom_name not in ("fx_filter", "inv_y", "fz_barrier"):
-        alg_step = kernel_alg_bytes(dom_name, N, 1, store) * dom_n
-    else:
-        launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
-        if dom_name == "fx_filter":
-            launches = [min(fx_nb, n - o) for n in launches for o in range(0, n, fx_nb)]
-        alg_step = sum(kernel_alg_bytes(dom_name, N, n, store) for n in launches)
-    alg = alg_step / dom_n
-    achieved = alg_step / (dom_ms * 1e-3) / 1e9 if alg_step else None
-    roof = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic_from_profiles(dom_name, args.workload, args.mode),
-            "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": alg,
+
+    def kernel_roof(name):
+        """Algorithmic bytes of a kernel over the step (the last fused batch may
+        carry fewer radii; the x pass fills a batch in launches of at most
+        FX_MAX_RADII, each reading the spectrum) over its event time."""
+        k_ms, k_n = kernels[name]
+        if args.mode == "plain" or name not in ("fx_filter", "inv_y", "fz_barrier"):
+            alg_step = kernel_alg_bytes(name, N, 1, store) * k_n
+        else:
+            launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
+            if name == "fx_filter":
+                launches = [min(fx_nb, n - o) for n in launches for o in range(0, n, fx_nb)]
+            alg_step = sum(kernel_alg_bytes(name, N, n, store) for n in launches)
+        achieved = alg_step / (k_ms * 1e-3) / 1e9 if alg_step else None
+        return {"achieved": achieved, "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                "traffic": traffic_from_profiles(name, args.workload, args.mode),
+                "avg_launch_ms": k_ms / k_n, "alg_bytes_per_launch": alg_step / k_n, "ms_per_step": k_ms}
+
+    dr = kernel_roof(dom_name)
+    roof = {"bound": "hbm", "kernel": dom_name, "achieved": dr["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dr["frac"], "traffic": dr["traffic"],
+            "avg_launch_ms": dr["avg_launch_ms"], "alg_bytes_per_launch": dr["alg_bytes_per_launch"],
             "b_alg_path": B_ALG_PATH[args.mode],
             "path_achieved": value * B_ALG_PATH[args.mode] / 1e9,
             "path_frac": value * B_ALG_PATH[args.mode] / 1e9 / HBM_PEAK_GBS}
+    # every fused pass, not only the dominant one (in the mixed path the x and
+    # y passes take about the same time per step)
+    if args.mode != "plain":
+        roof["fused_kernels"] = {k: kernel_roof(k) for k in ("fx_filter", "inv_y", "fz_barrier") if k in kernels}
     # measured HBM bytes of the whole step (rocprof FETCH_SIZE + WRITE_SIZE per
     # kernel launch, committed under profiles/) over the step time
     step_bytes = traffic_per_step(args.workload, args.mode, kernels)
@@ -373,10 +383,12 @@ def plain_fp32_rate(lp, steps: int) -> dict:
 
 
 def multibox(args):
-    """48 boxes of three sizes: one persistent launch per pass over all boxes
-    (prc_d2z_group_run: value), against the same boxes submitted together on
-    their own streams (prc_d2z_run_many) and one after another as separate
-    plain FP32 launches.  value = aggregate cell-radius updates per second."""
+    """48 boxes of three sizes as one group (prc_d2z_group_run: value): the
+    boxes spread over four chains (streams) by cells, each chain running its
+    boxes of one size through one launch per pass, one graph per step;
+    against the same boxes submitted together on their own streams
+    (prc_d2z_run_many) and one after another as separate plain FP32 launches.
+    value = aggregate cell-radius updates per second."""
     from prc_d2z import D2Z, D2ZParams, Group, run_many
 
     mode = MODES["fused"] if args.mode in ("mixed", "half") else MODES[args.mode]
@@ -423,7 +435,8 @@ def multibox(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "16x128^3 (200 Mpc/h) + 16x192^3 (300) + 16x256^3 (400), 48 radii each "
                                "1 cell..50 Mpc/h, distinct GRF seeds",
-                   "mode": "fused FP32 kernels, one persistent launch per pass over all 48 boxes"},
+                   "mode": "fused FP32 kernels; the boxes on four chains (streams) balanced by cells, "
+                           "one launch per pass for the boxes of one size on a chain, one graph per step"},
         "streams_ms": streams * 1e3,
         "speedup_vs_streams": streams / batched,
         "separate_plain_ms": seq_t * 1e3,

@@ -396,13 +396,13 @@ def test_c5_size_2048():
     assert min(float(z32[k:k + 64].min()) for k in range(0, N, 64)) >= 0.0
     # every 4th x plane (2.1e9 cells) keeps the host comparison to seconds
     d_plain = _chunked_max_abs_diff(out[PLAIN_FP32][0][::4], z32[::4])
-    d_mixed = _chunked_max_abs_diff(out[FUSED_HALF][0][::4], z32[::4])
+    d_half = _chunked_max_abs_diff(out[FUSED_HALF][0][::4], z32[::4])
     x32 = np.cumsum(h32.astype(np.int64)) / z32.size
     x16 = np.cumsum(out[FUSED_HALF][1].astype(np.int64)) / z32.size
-    print(f"2048^3 fused vs plain max|dz| {d_plain:.2e}; mixed vs fused max|dz| {d_mixed:.2e}, "
+    print(f"2048^3 fused vs plain max|dz| {d_plain:.2e}; half vs fused max|dz| {d_half:.2e}, "
           f"x_HII max diff {np.max(np.abs(x32 - x16)):.2e}")
     assert d_plain <= 1e-5
-    assert d_mixed <= 0.02
+    assert d_half <= 0.02
     assert np.max(np.abs(x32 - x16)) <= 1e-3
     assert int(h32.sum()) <= z32.size
 
