@@ -96,18 +96,20 @@ def traffic_from_profiles(kernel: str, workload: str, mode: str):
 
 def cpu_baseline_sample(delta: np.ndarray, lp, nthreads: int) -> dict:
     """Time the oracle-side CPU restatement on a bounded sample: the forward
-    transform plus one radius of the same box; the full-run rate is
-    extrapolated as N^3 Nscales / (t_forward + Nscales t_radius)."""
+    transform plus four radii spread over the schedule (about 14 s at 1024^3
+    on 16 threads); the full-run rate is extrapolated as
+    N^3 Nscales / (t_forward + Nscales mean(t_radius))."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import d2z_cpu  # oracle-side C restatement (float, OpenMP)
 
     N, NS = lp.N, lp.Nscales
-    res = d2z_cpu.time_sample(delta, lp, radii=[NS // 2], nthreads=nthreads)
+    radii = sorted({0, NS // 3, (2 * NS) // 3, NS - 1})
+    res = d2z_cpu.time_sample(delta, lp, radii=radii, nthreads=nthreads)
     t_fwd, t_rad = res["t_forward"], res["t_radius"]
     rate = N ** 3 * NS / (t_fwd + NS * t_rad)
     return {"value": rate, "unit": "cell*radius/s", "cores": nthreads, "kind": "port",
-            "sample": f"{N}^3 forward r2c + 1 of {NS} radii (r={NS // 2}) with the C FP32 restatement "
-                      f"({res['fft']} FFT); t_forward={t_fwd:.2f}s t_radius={t_rad:.2f}s, "
+            "sample": f"{N}^3 forward r2c + {len(radii)} of {NS} radii (r={radii}) with the C FP32 restatement "
+                      f"({res['fft']} FFT); t_forward={t_fwd:.2f}s mean t_radius={t_rad:.2f}s, "
                       f"rate extrapolated to {NS} radii",
             "host": host_cpu()}
 
@@ -264,14 +266,17 @@ def main():
         roof["path_traffic_bytes"] = step_bytes
         roof["path_traffic_frac"] = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
 
-    # the same steps replayed as the captured hipGraph (no per-launch events)
+    # the same steps replayed as the captured hipGraph (no per-launch events),
+    # each step timed on its own for the median
     d.run()
     d.sync()
-    g0 = time.perf_counter()
+    step_ms = []
     for _ in range(args.steps):
+        g0 = time.perf_counter()
         d.run_async()
-    d.sync()
-    graph_ms = (time.perf_counter() - g0) / args.steps * 1e3
+        d.sync()
+        step_ms.append((time.perf_counter() - g0) * 1e3)
+    graph_ms = sum(step_ms) / len(step_ms)
 
     if args.history:
         h = d.history()
@@ -309,6 +314,7 @@ def main():
                                   "z": ("matrix" if plan["z_plan"] else "vector")} if plan else None),
                    "autotuned": tuning is not None},
         "graph_ms_per_step": graph_ms,
+        "graph_ms_per_step_median": float(np.median(step_ms)),
         "roofline": roof,
         "kernels_ms_last_step": {k: v[0] for k, v in kernels.items()},
         "plain_fp32": plain,
