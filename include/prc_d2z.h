@@ -281,8 +281,10 @@ int prc_d2z_run_many(prc_d2z_ctx* const* ctxs, int n);
  * captured graph.  The boxes keep their contexts (create, set_delta or
  * generate_delta, get_zreion as usual); results equal each box's prc_d2z_run
  * bit for bit (sigma_grid may differ in the last bits: its partial sums are
- * grouped differently).  No counterpart in the reference, which runs one box
- * per process. */
+ * grouped differently).  The group holds the contexts (and the buffers and
+ * launch shapes they had at create): destroy it before any of them, and
+ * create a new group after changing a box's launch configuration.  No
+ * counterpart in the reference, which runs one box per process. */
 typedef struct prc_d2z_group prc_d2z_group;
 int prc_d2z_group_create(prc_d2z_ctx* const* ctxs, int n, prc_d2z_group** out);
 int prc_d2z_group_run(prc_d2z_group* grp);

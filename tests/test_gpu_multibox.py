@@ -106,3 +106,39 @@ def test_group_with_bubble_radii_and_errors():
             Group([boxes[0], other])
     for d in boxes:
         d.close()
+
+
+def test_group_of_one_size_and_of_one_box():
+    """Seven 64^3 boxes (the chains split one size: 2 + 2 + 2 + 1 boxes) and a
+    group of a single box: every zreion equals the box's own run bit for bit,
+    and sigma_grid agrees with it to its summation order."""
+    boxes = []
+    for k in range(7):
+        d = D2Z(_params(64, 100.0, nscales=20))
+        d.generate_delta(40 + k)
+        boxes.append(d)
+    ref = []
+    for d in boxes:
+        d.run()
+        ref.append((d.zreion(), d.radius_table()))
+    for d in boxes:  # clobber the results
+        d.generate_delta(999)
+        d.run()
+    for k, d in enumerate(boxes):
+        d.generate_delta(40 + k)
+    announce("group of 7 boxes of 64^3")
+    with Group(boxes) as g:
+        g.run()
+        for k, (d, (z, tab)) in enumerate(zip(boxes, ref)):
+            assert np.array_equal(d.zreion(), z), k
+            np.testing.assert_allclose(d.radius_table()[:, 2], tab[:, 2], rtol=1e-6)
+    announce("group of one box")
+    with Group(boxes[3:4]) as g:
+        boxes[3].generate_delta(999)
+        boxes[3].run()  # another field's zreion, so that a group run that did nothing shows
+        assert not np.array_equal(boxes[3].zreion(), ref[3][0])
+        boxes[3].generate_delta(43)
+        g.run()
+        assert np.array_equal(boxes[3].zreion(), ref[3][0])
+    for d in boxes:
+        d.close()
