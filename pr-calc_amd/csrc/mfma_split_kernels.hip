@@ -136,11 +136,13 @@ __device__ __forceinline__ float4_t mfma_split(const SplitOp& a, const SplitOp& 
 // once per launch so that |filtered value| <= 2^10; the radix-16 stage grows
 // values by at most 16 sqrt 2, so the second stage's FP16 parts stay below
 // 2^15.  The outputs are multiplied by 1/bs (exact) before they are stored.
-// The low parts keep their 2^11 scale and their own accumulator: with the low
-// parts unscaled and the three products in one register (normal FP16 for
-// values above 2^-3 of this scale) the pass measured 1.5% faster, but the
-// largest error of a 1024^3 smoothed field rose from 2.7e-6 to 5.8e-5 of its
-// rms (the rms error unchanged).
+// The two correction products keep their own accumulator and join the main
+// product in one FP32 FMA.  Accumulating all three products in one MFMA
+// accumulator measured 1-1.5% faster but raised the largest error of a 1024^3
+// smoothed field from 2.7e-6 to 5.3e-5 - 5.8e-5 of its rms, whether the low
+// parts were left unscaled or the matrices (x 2^5) and the data (<= 2^5) were
+// scaled so that every part is a normal FP16 number: the matrix cores' sum
+// of the large products and the small accumulator input is not FP32-exact.
 //
 // LDS exchange rows hold 8 columns of float2 (64 B); row `row` is stored at
 // row ^ bit4(row), which puts the two halves of every quarter wave (butterfly
