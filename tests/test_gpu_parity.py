@@ -237,6 +237,41 @@ def test_field_generator_matches_oracle():
     assert np.max(np.abs(g - ref)) <= 2e-5 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32, FUSED_MIXED, FUSED_HALF])
+def test_c1_config_128(mode):
+    """The C1 configuration as stated: 128^3, 200 Mpc/h, 32 radii from one cell
+    (1.5625 Mpc/h) to 50 Mpc/h, zeta 40, M_min 1e8 Msun/h, the GPU field of
+    seed 1; every path's zreion over all 32 radii against the FP64 oracle (FP32
+    paths: max |dz| <= 2e-4, no cell above 0.01; half path: max |dz| <= 0.02,
+    fraction above 0.01 <= 1e-3; x_HII within 1e-3) and sigma_grid per radius."""
+    from prc_d2z import D2ZParams
+    N = 128
+    lp = D2ZParams(N=N, BoxSize=200.0, Nscales=32, Rmax=50.0, zeta=40.0, M_min=1.4706e8, mode=mode)
+    announce(f"C1 128^3 x 32 mode={mode}")
+    with D2Z(lp) as d:
+        d.generate_delta(1)
+        delta = d.delta()
+        d.run()
+        z = d.zreion()
+        tab = d.radius_table()
+    key = ("c1", N)
+    if key not in _oracle_cache:
+        op = oracle_params(lp)
+        sc = oracle.setup_scalars(op)
+        _oracle_cache[key] = (sc,) + tuple(oracle.delta2zreion(delta, op, sc)[:2])
+    sc, zr, sg = _oracle_cache[key]
+    m = zreion_metrics(z, zr)
+    print(f"C1 mode {mode}: {m}")
+    if mode == FUSED_HALF:
+        assert m["max_abs"] <= 0.02 and m["frac_gt_0.01"] <= 1e-3
+        np.testing.assert_allclose(tab[:, 2], np.array(sg, dtype=np.float64), rtol=2e-3)
+    else:
+        assert m["max_abs"] <= TOL_Z and m["frac_gt_0.01"] == 0.0
+        np.testing.assert_allclose(tab[:, 2], np.array(sg, dtype=np.float64), rtol=1e-5)
+    assert m["xhii_max"] <= 1e-3
+    assert np.mean(z > 0) > 0.05
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("mode", [PLAIN_FP32, FUSED_FP32])
 def test_c2_size_512(mode):
