@@ -248,9 +248,11 @@ int prc_d2z_get_launch_config(const prc_d2z_ctx* ctx, int* radii_per_launch, int
 
 /* FFT plans of the fused x and z passes: the vector radix plans (radix 2, 3,
  * 4, 8, 16 stages on the vector ALUs) or the matrix-core plans (radix-16
- * stages on the matrix cores: split hi/lo FP16 operands in the mixed mode,
- * plain FP16 in the half mode; N = 1024).  Default: matrix-core where the
- * mode has them.  x_radii_per_launch: radii per x-pass launch, 1..16. */
+ * stages on the matrix cores: split hi/lo FP16 operands in the mixed mode at
+ * N = 1024; plain FP16 in the half mode, x pass at N = 1024 and 2048, z pass
+ * at N = 1024).  Default: matrix-core where the mode and N have them; asking
+ * for one that does not exist returns UNSUPPORTED_N.  x_radii_per_launch:
+ * radii per x-pass launch, 1..16. */
 enum prc_d2z_plan { PRC_D2Z_PLAN_VECTOR = 0, PRC_D2Z_PLAN_MATRIX = 1 };
 int prc_d2z_set_plan(prc_d2z_ctx* ctx, int x_plan, int z_plan, int x_radii_per_launch);
 int prc_d2z_get_plan(const prc_d2z_ctx* ctx, int* x_plan, int* z_plan, int* x_radii_per_launch);

@@ -234,6 +234,173 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// ------------------------------------------------------ x pass, N = 2048
+// The same pass for 2048 = 16 * 16 * 8 (FP16 intermediates): the two radix-16
+// stages on the matrix cores with the same operand matrices (they depend on
+// the stage's p only), the radix-8 stage (p = 256) on the vector ALUs.  The
+// exchange buffers of 2048 rows leave room for 4 of the 8 kz columns of a
+// spectrum tile (C = 8 at N = 2048) per block: MFMA column n = (column c = n
+// & 3, butterfly sel = n >> 2).  Stage A: butterflies b = 8 w + 4 t + sel,
+// inputs r = 4 g + j at kx = b + 128 r, outputs at 16 b + q.  Stage B: k =
+// wave w, butterflies b' = w + 16 (4 t + sel), inputs at b' + 128 r, outputs at
+// 256 (b' / 16) + w + 16 q.  Stage C: one radix-8 butterfly b2 = tid / 4 of
+// column tid % 4 per thread, inputs at b2 + 256 r, twiddles w2048^(r b2) from
+// the tile plan's stage table, outputs in natural order at b2 + 256 q.  |k| of
+// the W table's entries is recomputed per radius (no |k| table: the W(x)
+// table may take 56 KB of LDS here).
+namespace x2048 {
+constexpr int kN = 2048;
+constexpr int kC = 8;            // Blk<2048>::C
+constexpr int kCF = 4;           // columns per block
+constexpr int kNSUB = kC / kCF;
+constexpr int kNF = kN / 2 + 1;
+constexpr int kThreads = 1024;
+// Row swizzles (XOR of row bits 0-2, a bijection inside every 8 rows) that put
+// the 32 lanes of every half-wave access on distinct banks: bufX is written
+// at rows 16 b + q (lanes differ in q's bits 0-1 and b's bits 0-1 = rows' bits
+// 4-5) and read at rows b' + 128 r (lanes differ in rows' bits 4-5 and 9);
+// bufY is written at rows 256 j + k + 16 q (bits 4 and 8-9 vary) and read at
+// rows b2 + 256 r (bits 0-2 vary).
+__device__ __forceinline__ int xsw(int row) { return row ^ ((((row >> 4) & 3) << 1) ^ ((row >> 9) & 1)); }
+__device__ __forceinline__ int ysw(int row) { return row ^ (((row >> 4) & 1) | (((row >> 8) & 3) << 1)); }
+}  // namespace x2048
+
+__global__ void __launch_bounds__(x2048::kThreads)
+    k_fx_mfma2048(const float2* __restrict__ S, const float2* __restrict__ D0, const float2* __restrict__ DN,
+                  __half2* __restrict__ Tbuf, float dk, float invN3, const RadiusConst* __restrict__ rc, int r0,
+                  int nb, const float2* __restrict__ tw_plan, const float4* __restrict__ wcoef, int wM, float winvh,
+                  const half8_t* __restrict__ mats) {
+  constexpr int N2 = x2048::kN, C2 = x2048::kC, CF2 = x2048::kCF, NSUB2 = x2048::kNSUB, NF2 = x2048::kNF;
+  constexpr int T2 = x2048::kThreads;
+  using x2048::xsw;
+  using x2048::ysw;
+  constexpr long long plane = (long long)N2 * N2 * (N2 / 2);
+  __shared__ uint32_t bufX[N2 * CF2];
+  __shared__ uint32_t bufY[N2 * CF2];
+  __shared__ float wtab[(CF2 + 1) * NF2];  // [|fx|][column], then the kz = N/2 column
+  extern __shared__ float4 wsh[];           // W(x) interpolation table (wM intervals)
+  float* wpk = wtab + CF2 * NF2;
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63;
+  const int g = l >> 4, n = l & 15;
+  const int c = n & 3, sel = n >> 2;
+
+  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int sub = (int)(tile % NSUB2);
+  const int ky = (int)((tile / NSUB2) % N2);
+  const int kzb = (int)(tile / (NSUB2 * N2));
+  const int kz0 = kzb * C2 + sub * CF2;
+  const bool has_packed = (kzb == 0 && sub == 0);  // block-uniform
+  const bool packed = has_packed && c == 0;        // this lane's column is kz = 0
+
+  const half8_t gA0 = mats[mat_set_a(0) * 64 + l], gA1 = mats[mat_set_a(1) * 64 + l];
+  const half8_t gB0 = mats[mat_set_b(w, 0) * 64 + l], gB1 = mats[mat_set_b(w, 1) * 64 + l];
+
+  const float2* sp = S + ((long long)kzb * N2 + ky) * N2 * C2 + sub * CF2 + c;
+  float2 m[8];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kx = 8 * w + 4 * t + sel + 128 * (4 * g + j);
+      m[4 * t + j] = packed ? D0[(long long)kx * N2 + ky] : sp[(long long)kx * C2];
+    }
+  }
+  const int fy2 = fsq(ky, N2);
+  const int ntab = (has_packed ? CF2 + 1 : CF2) * NF2;
+  for (int q = tid; q < wM; q += T2) wsh[q] = wcoef[q];
+  const int cl = tid & 3, b2 = tid >> 2;  // stage C: column cl, butterfly b2 < 256
+  __half2* tp = Tbuf + (long long)kzb * N2 * N2 * C2 + (long long)ky * C2 + sub * CF2 + cl;
+  __syncthreads();
+
+#pragma unroll 1
+  for (int rl = 0; rl < nb; ++rl) {
+    // opaque zero: the radius-invariant LDS and store addresses and the
+    // stage-C twiddles are rebuilt per radius instead of held in registers
+    int zo = 0;
+    asm volatile("" : "+v"(zo));
+    const float dkR = dk * rc[r0 + rl].R;
+    const float dkRh = dkR * winvh;
+    for (int q = tid; q < ntab; q += T2) {
+      int f, kzq;
+      if (q < CF2 * NF2) {
+        f = q / CF2;
+        kzq = kz0 + q % CF2;
+      } else {
+        f = q - CF2 * NF2;
+        kzq = N2 / 2;
+      }
+      wtab[q] = filter_from_table(kmag_grid(f * f + fy2 + kzq * kzq), dkR, dkRh, invN3, wsh, wM);
+    }
+    __syncthreads();  // (1) W table complete; bufX's stage-B reads of the last radius done
+    // stage A: filter, radix 16 on the matrix cores, outputs to bufX
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int b = 8 * w + 4 * t + sel;
+      float2 x[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int kx = b + 128 * (4 * g + j);
+        const int fa = (kx <= N2 / 2) ? kx : N2 - kx;
+        const float w0 = wtab[fa * CF2 + c];
+        if (!packed) {
+          x[j] = cscale(m[4 * t + j], w0);
+        } else {
+          const float wn = wpk[fa];
+          const float2 dn = DN[(long long)kx * N2 + ky];
+          x[j] = make_float2(w0 * m[4 * t + j].x - wn * dn.y, w0 * m[4 * t + j].y + wn * dn.x);
+        }
+      }
+      const half8_t bop = pack_b(x);
+      const float4_t z = {0.f, 0.f, 0.f, 0.f};
+      const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(gA0, bop, z, 0, 0, 0);
+      const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(gA1, bop, z, 0, 0, 0);
+      // outputs q = g, g + 4 (block 0), g + 8, g + 12 (block 1) at 16 b + q
+      const int row = 16 * b + g + zo;
+      bufX[xsw(row) * CF2 + c] = pack_h2(d0[0], d0[1]);
+      bufX[xsw(row + 4) * CF2 + c] = pack_h2(d0[2], d0[3]);
+      bufX[xsw(row + 8) * CF2 + c] = pack_h2(d1[0], d1[1]);
+      bufX[xsw(row + 12) * CF2 + c] = pack_h2(d1[2], d1[3]);
+    }
+    __syncthreads();  // (2) stage A's outputs complete
+    // stage B: k = w; butterflies b' = w + 16 (4 t + sel), inputs at b' + 128 r
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int bp = w + 16 * (4 * t + sel);
+      union {
+        uint32_t u[4];
+        half8_t h;
+      } cv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cv.u[j] = bufX[xsw(bp + zo + 128 * (4 * g + j)) * CF2 + c];
+      const float4_t z = {0.f, 0.f, 0.f, 0.f};
+      const float4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(gB0, cv.h, z, 0, 0, 0);
+      const float4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(gB1, cv.h, z, 0, 0, 0);
+      // output q at 256 (b' / 16) + w + 16 q
+      const int row = (bp >> 4) * 256 + w + 16 * g + zo;
+      bufY[ysw(row) * CF2 + c] = pack_h2(d0[0], d0[1]);
+      bufY[ysw(row + 64) * CF2 + c] = pack_h2(d0[2], d0[3]);
+      bufY[ysw(row + 128) * CF2 + c] = pack_h2(d1[0], d1[1]);
+      bufY[ysw(row + 192) * CF2 + c] = pack_h2(d1[2], d1[3]);
+    }
+    __syncthreads();  // (3) stage B's outputs complete
+    // stage C: radix 8, p = 256, natural-order outputs
+    __half2* dst = tp + (long long)rl * plane;
+    float2 u[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) u[r] = unpack_h2(bufY[ysw(b2 + zo + 256 * r) * CF2 + cl]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) u[r] = cmul(u[r], tw_plan[256 + 8 * b2 + r + zo]);
+    DFT<8, +1>::run(u);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[(long long)(b2 + zo + 256 * q) * N2 * C2] = __float22half2_rn(u[q]);
+    // next radius: wtab was last read in stage A and bufX in stage B, both
+    // before a barrier; bufY (read just above) is rewritten in stage B only
+    // after the barriers that follow the table fill and stage A
+  }
+}
+
 // ---------------------------------------------------------------- z pass
 // Last inverse pass (z, c2r) of the mixed path at N = 1024 with the barrier
 // fused, as k_fz (reference delta2zreion_d2z.C:92-146,206-216), for blocks of
@@ -477,11 +644,19 @@ hipError_t launch_fz_mfma(int N, const void* Tbuf, float* zre, float* rbub, cons
   return hipGetLastError();
 }
 
-bool fx_mfma_supported(int N, bool half) { return half && N == kN; }
+bool fx_mfma_supported(int N, bool half) { return half && (N == kN || N == x2048::kN); }
 
 hipError_t launch_fx_mfma(int N, const float2* S, const float2* D0, const float2* DN, void* Tbuf, float dk,
                           float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw_full_plan,
                           const WTable& wt, const void* mats, hipStream_t s) {
+  if (N == x2048::kN) {
+    if (nb < 1 || nb > kFxMaxRadii || wt.M < 0 || wt.M > kWTableMax || !mats) return hipErrorInvalidValue;
+    const long long nblk = (long long)x2048::kN * ((x2048::kN / 2) / x2048::kC) * x2048::kNSUB;
+    hipLaunchKernelGGL(k_fx_mfma2048, dim3((unsigned)nblk), dim3(x2048::kThreads), sizeof(float4) * wt.M, s, S, D0,
+                       DN, static_cast<__half2*>(Tbuf), dk, invN3, rc, r0, nb, tw_full_plan, wt.coef, wt.M, wt.invh,
+                       static_cast<const half8_t*>(mats));
+    return hipGetLastError();
+  }
   if (N != kN || nb < 1 || nb > kFxMaxRadii || wt.M < 0 || wt.M > kWTableMax || !mats) return hipErrorInvalidValue;
   const long long nblk = (long long)kN * ((kN / 2) / kC) * kNSUB;
   hipLaunchKernelGGL(k_fx_mfma, dim3((unsigned)nblk), dim3(kThreads), sizeof(float4) * wt.M, s, S, D0, DN,

@@ -84,11 +84,11 @@ bool supported_n(int N) {
 bool half_storage(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_HALF; }
 // the compensated split-FP16 matrix-core x and z passes (mixed mode, FP32 storage)
 bool use_split(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_MIXED && split_supported(p.N); }
-// a matrix-core plan exists for the mode and N: the split passes (mixed) or the
-// FP16 ones (half); otherwise both passes run their vector plans
-bool has_matrix_plan(const prc_d2z_params& p) {
-  return use_split(p) || (half_storage(p) && fx_mfma_supported(p.N, true) && fz_mfma_supported(p.N, true));
-}
+// a matrix-core plan exists for the mode, N and pass: the split passes (mixed,
+// N = 1024) or the FP16 ones (half: x at N = 1024 and 2048, z at 1024);
+// otherwise the pass runs its vector plan
+bool has_matrix_x(const prc_d2z_params& p) { return use_split(p) || (half_storage(p) && fx_mfma_supported(p.N, true)); }
+bool has_matrix_z(const prc_d2z_params& p) { return use_split(p) || (half_storage(p) && fz_mfma_supported(p.N, true)); }
 
 }  // namespace
 
@@ -217,11 +217,11 @@ int forward(prc_d2z_ctx* c, float dinit) {
 // grid of the fused z pass: the resident capacity of the kernel the plan uses
 int default_fz_blocks(const prc_d2z_params& p, int zplan) {
   const bool half = half_storage(p), rb = p.RecordBubbleSizes != 0;
-  if (zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(p)) return use_split(p) ? fz_split_blocks(rb) : fz_mfma_blocks(rb);
+  if (zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_z(p)) return use_split(p) ? fz_split_blocks(rb) : fz_mfma_blocks(rb);
   return fz_blocks(p.N, half, rb);
 }
-bool matrix_x(const prc_d2z_ctx* c) { return c->xplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(c->p); }
-bool matrix_z(const prc_d2z_ctx* c) { return c->zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_plan(c->p); }
+bool matrix_x(const prc_d2z_ctx* c) { return c->xplan == PRC_D2Z_PLAN_MATRIX && has_matrix_x(c->p); }
+bool matrix_z(const prc_d2z_ctx* c) { return c->zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_z(c->p); }
 
 // BoxSize in Mpc as the d2z program holds it after Parameters.BoxSize /= h
 // (main_d2z.C:29, float division)
@@ -628,7 +628,8 @@ int prc_d2z_create(const prc_d2z_params* p, prc_d2z_ctx** out) {
   if (hipSetDevice(p->device) != hipSuccess) return bail(PRC_D2Z_ERR_HIP, "hipSetDevice failed");
   // memory budget check before allocating
   const int NS = p->Nscales;
-  if (has_matrix_plan(*p)) c->xplan = c->zplan = PRC_D2Z_PLAN_MATRIX;
+  if (has_matrix_x(*p)) c->xplan = PRC_D2Z_PLAN_MATRIX;
+  if (has_matrix_z(*p)) c->zplan = PRC_D2Z_PLAN_MATRIX;
   c->nblk_fz = default_fz_blocks(*p, c->zplan);
   // intermediates: the fused batch (FP32 or FP16 storage), and at least two
   // FP32-sized buffers for the forward scratch, the plain path and smoothed_field
@@ -1321,12 +1322,12 @@ int prc_d2z_get_launch_config(const prc_d2z_ctx* c, int* radii_per_launch, int* 
 int prc_d2z_set_plan(prc_d2z_ctx* c, int x_plan, int z_plan, int x_radii_per_launch) {
   if (!c) return PRC_D2Z_ERR_BAD_ARG;
   if (c->p.mode == PRC_D2Z_PLAIN_FP32) return c->fail(PRC_D2Z_ERR_STATE, "plans apply to fused modes");
-  for (int pl : {x_plan, z_plan}) {
+  for (int pl : {x_plan, z_plan})
     if (pl != PRC_D2Z_PLAN_VECTOR && pl != PRC_D2Z_PLAN_MATRIX)
       return c->fail(PRC_D2Z_ERR_BAD_ARG, "plan must be PRC_D2Z_PLAN_VECTOR or PRC_D2Z_PLAN_MATRIX");
-    if (pl == PRC_D2Z_PLAN_MATRIX && !has_matrix_plan(c->p))
-      return c->fail(PRC_D2Z_ERR_UNSUPPORTED_N, "no matrix-core plan for this mode and N (mixed and half, N = 1024)");
-  }
+  if ((x_plan == PRC_D2Z_PLAN_MATRIX && !has_matrix_x(c->p)) || (z_plan == PRC_D2Z_PLAN_MATRIX && !has_matrix_z(c->p)))
+    return c->fail(PRC_D2Z_ERR_UNSUPPORTED_N,
+                   "no matrix-core plan for this pass, mode and N (mixed: N = 1024; half: x at 1024 and 2048, z at 1024)");
   if (x_radii_per_launch < 1 || x_radii_per_launch > kFxMaxRadii)
     return c->fail(PRC_D2Z_ERR_BAD_ARG, "x_radii_per_launch must be in 1.." + std::to_string(kFxMaxRadii));
   if (z_plan != c->zplan) c->nblk_fz = default_fz_blocks(c->p, z_plan);
@@ -1370,8 +1371,9 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
   const int nr = std::min(c->p.Nscales, c->nbatch_cap);
   std::vector<int> nbs = {c->nbatch_cap};
   if (c->nbatch_cap >= 2) nbs.push_back(c->nbatch_cap / 2);
-  std::vector<int> plans = {PRC_D2Z_PLAN_VECTOR};
-  if (has_matrix_plan(c->p)) plans.push_back(PRC_D2Z_PLAN_MATRIX);
+  std::vector<int> xplans = {PRC_D2Z_PLAN_VECTOR}, zplans = {PRC_D2Z_PLAN_VECTOR};
+  if (has_matrix_x(c->p)) xplans.push_back(PRC_D2Z_PLAN_MATRIX);
+  if (has_matrix_z(c->p)) zplans.push_back(PRC_D2Z_PLAN_MATRIX);
   const std::vector<int> fxs = {8, kFxMaxRadii};
   hipEvent_t a = nullptr, b = nullptr;
   (void)hipEventCreate(&a);
@@ -1379,8 +1381,8 @@ int prc_d2z_autotune(prc_d2z_ctx* c, prc_d2z_tuning* out) {
   double best = 1e30;
   prc_d2z_tuning bt{c->nbatch, c->nblk_fz, 0.0, c->xplan, c->zplan, c->fx_nb};
   const prc_d2z_tuning keep = bt;
-  for (int xp : plans)
-    for (int zp : plans)
+  for (int xp : xplans)
+    for (int zp : zplans)
       for (int fx : fxs)
         for (int nb : nbs) {
           const int g0 = default_fz_blocks(c->p, zp);

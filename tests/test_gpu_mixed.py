@@ -14,7 +14,7 @@ import pytest
 import d2z_fp64 as oracle
 from _cases import field, lib_params, oracle_params, zreion_metrics
 from conftest import announce
-from prc_d2z import D2Z, D2ZParams, FUSED_FP32, FUSED_HALF
+from prc_d2z import D2Z, D2ZError, D2ZParams, FUSED_FP32, FUSED_HALF
 
 pytestmark = pytest.mark.gpu
 
@@ -144,3 +144,39 @@ def test_half_rbubble_1024():
     assert np.all((rb16 == 0) == (z16 == 0))
     assert mism <= 0.05
     assert dz[mis].max() <= 0.02  # the maximum agrees where the radius moved
+
+
+def test_half_matrix_x_2048():
+    """At 2048^3 the half path's x pass runs on the matrix cores (2048 = 16 x
+    16 x 8, 4 of the 8 columns of a spectrum tile per block).  The smoothed
+    field of the smallest radius through that pass and through the vector pass,
+    each against the fused FP32 path's (every 8th x plane): relative rms error
+    <= 2e-3 and largest error <= 2e-2 of the rms, the half path's bounds."""
+    from prc_d2z import PLAN_MATRIX, PLAN_VECTOR
+    N = 2048
+    r0 = 4000.0 / N
+    mk = lambda mode: D2ZParams(N=N, BoxSize=4000.0, Nscales=4, Rmax=r0 * (150.0 / r0) ** (3 / 127), zeta=40.0,
+                                M_min=1.4706e8, mode=mode)
+    fields = {}
+    for key, mode, xplan in (("fp32", FUSED_FP32, None), ("matrix", FUSED_HALF, PLAN_MATRIX),
+                             ("vector", FUSED_HALF, PLAN_VECTOR)):
+        announce(f"2048^3 smoothed field, {key} x pass")
+        with D2Z(mk(mode)) as d:
+            d.generate_delta(1)
+            if xplan is not None:
+                if xplan == PLAN_MATRIX:  # the default plan at this size; no matrix-core z pass here
+                    assert d.plan() == {"x_plan": PLAN_MATRIX, "z_plan": PLAN_VECTOR, "x_radii_per_launch": 16}
+                    with pytest.raises(D2ZError):
+                        d.set_plan(PLAN_MATRIX, PLAN_MATRIX, 16)
+                d.set_plan(xplan, PLAN_VECTOR, 16)
+                assert d.plan()["x_plan"] == xplan
+            f = d.smoothed_field(0)
+            fields[key] = f[::8].astype(np.float64)
+            del f
+    ref = fields.pop("fp32")
+    rms = np.sqrt(np.mean(ref ** 2))
+    for key, g in fields.items():
+        e = g - ref
+        err_rms, err_max = np.sqrt(np.mean(e ** 2)) / rms, np.max(np.abs(e)) / rms
+        print(f"2048^3 half path, {key} x pass: delta_R rel rms err {err_rms:.2e}, rel max err {err_max:.2e}")
+        assert err_rms <= 2e-3 and err_max <= 2e-2
