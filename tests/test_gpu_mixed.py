@@ -1,7 +1,8 @@
 """The half path (FUSED_HALF: FP32 arithmetic, FP16 storage of the two
-inter-pass intermediates, FP16 matrix-core stages at N = 1024) against the
-FP64 oracle.  The compensated mixed path (FUSED_MIXED) at 1024^3 is checked in
-test_gpu_parity.py::test_c3_size_1024 to FP32 tolerances.
+inter-pass intermediates, FP16 matrix-core stages at N = 1024 and in the x
+pass at N = 2048) against the FP64 oracle.  The compensated mixed path
+(FUSED_MIXED) at 1024^3 is checked in test_gpu_parity.py::test_c3_size_1024
+to FP32 tolerances.
 
 Tolerances (stated for the half path): max |dzreion| <= 0.02, the fraction of
 cells with |dzreion| > 0.01 at most 1e-3, x_HII(z) within 1e-3; the per-radius
