@@ -145,11 +145,12 @@ hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN,
                      float invN3, const RadiusConst* rc, int r0, int nb, const float2* tw, const WTable& wt,
                      hipStream_t s);
 // blocks of the fused z pass: the device's resident capacity for that kernel,
-// at most kFzMaxBlocks and at most one per 256-thread row group
+// at most kFzMaxBlocks and at most one per 256-thread row group.  fbar (FP32
+// storage only): the barrier as one FP32 FMA instead of the double evaluation
 constexpr int kFzMaxBlocks = 4096;
-int fz_blocks(int N, bool half, bool rb);
+int fz_blocks(int N, bool half, bool rb, bool fbar);
 hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
-                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf, bool fbar,
                      hipStream_t s);
 // ------------------------------------------------------ multi-box launches
 // One box of a persistent multi-box launch (the buffers of its context).

@@ -570,8 +570,10 @@ __device__ __forceinline__ void fz_tables(double (*red)[kFusedMaxRadii], float2*
 // the maximum over the batch stays in registers (first radius wins ties) and
 // zreion (and rbubble) is read and written once.  Per-radius sums of
 // delta_R^2 (float products, double sums as GetMeanSigma; with FP16 storage
-// the wave's own sum is float) are added per wave into red.
-template <class P, int C, bool RB, class TS>
+// the wave's own sum is float) are added per wave into red.  F32B: the
+// barrier as one FP32 FMA (barrier_z_f32), as for FP16 storage, instead of the
+// reference's double evaluation (the mixed path's FP32 intermediates).
+template <class P, int C, bool RB, class TS, bool F32B = false>
 __device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __restrict__ zre,
                                          float* __restrict__ rbub, long long nrows,
                                          const RadiusConst* __restrict__ rc, int r0, int nb, float iscale,
@@ -630,16 +632,16 @@ __device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __r
     }
     pencil_fft<P, +1, 0, Lds, true>(v, i, s_twh + zero, lds);
     // the output scale iscale (a power of two) is applied inside the
-    // barrier's FMA and to the sum of squares, where it is exact
-    // (the FP16-storage path evaluates it in FP32, far inside its own error)
+    // barrier's FMA and to the sum of squares, where it is exact (the FP16
+    // storage and F32B evaluate it in FP32: |error| <= (|z| + |A B - 1|) 2^-24)
     const double A = rc[r0 + q].A, B = rc[r0 + q].B;
     const BarrierF32 bk = barrier_f32(A, B, iscale);
     const float R = rc[r0 + q].R;
     float acc = 0.0f;  // this thread's 2E cells; blocks and waves are summed in double
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const float z0 = kH ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
-      const float z1 = kH ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
+      const float z0 = (kH || F32B) ? barrier_z_f32(v[e].x, bk) : barrier_z_scaled(v[e].x, sc, A, B);
+      const float z1 = (kH || F32B) ? barrier_z_f32(v[e].y, bk) : barrier_z_scaled(v[e].y, sc, A, B);
       acc = fmaf(v[e].x, v[e].x, fmaf(v[e].y, v[e].y, acc));
       if constexpr (RB) {
         if (z0 > best[e].x) {
@@ -680,7 +682,7 @@ __device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __r
 
 // The fused z pass (c2r + barrier + running max) over all rows: a
 // grid-stride kernel over row groups; one partial sum per block and radius.
-template <class P, int C, bool RB, class TS>
+template <class P, int C, bool RB, class TS, bool F32B = false>
 __global__ void __launch_bounds__(256)
     k_fz(const TS* __restrict__ Tbuf, float* __restrict__ zre, float* __restrict__ rbub, long long nrows,
          const RadiusConst* __restrict__ rc, int r0, int nb, double* __restrict__ partial, float iscale,
@@ -698,7 +700,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
 #pragma unroll 1
   for (long long base = (long long)blockIdx.x * ROWS; base < nrows; base += (long long)gridDim.x * ROWS)
-    fz_group<P, C, RB, TS>(Tbuf, zre, rbub, nrows, rc, r0, nb, iscale, base, s_twh, s_twf, smem, red);
+    fz_group<P, C, RB, TS, F32B>(Tbuf, zre, rbub, nrows, rc, r0, nb, iscale, base, s_twh, s_twf, smem, red);
   __syncthreads();
   const int t = threadIdx.x;
   if (t < nb) {
@@ -1290,7 +1292,7 @@ static hipError_t fx_impl(const float2* S, const float2* D0, const float2* DN, T
 }
 
 template <int N>
-static int fz_grid(bool half, bool rb) {
+static int fz_grid(bool half, bool rb, bool fbar) {
   // grid-stride kernel: one block per resident slot of the device (all blocks
   // run in one wave of the grid), never more blocks than row groups
   using P = Plan<N / 2>;
@@ -1304,6 +1306,9 @@ static int fz_grid(bool half, bool rb) {
     if (half)
       e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, true, __half2>, 256, 0)
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, false, __half2>, 256, 0);
+    else if (fbar)
+      e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, true, float2, true>, 256, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, false, float2, true>, 256, 0);
     else
       e = rb ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, true, float2>, 256, 0)
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fz<P, C, false, float2>, 256, 0);
@@ -1315,11 +1320,22 @@ static int fz_grid(bool half, bool rb) {
 
 template <int N, class TS>
 static hipError_t fz_impl(const TS* Tbuf, float* zre, float* rbub, const RadiusConst* rc, int r0, int nb,
-                          double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                          double* partial, int nblk, float iscale, const float2* twh, const float2* twf, bool fbar,
                           hipStream_t s) {
   using P = Plan<N / 2>;
   constexpr int C = Blk<N>::C;
   const long long nrows = (long long)N * N;
+  if constexpr (std::is_same<TS, float2>::value) {
+    if (fbar) {
+      if (rbub)
+        hipLaunchKernelGGL((k_fz<P, C, true, TS, true>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0,
+                           nb, partial, iscale, twh, twf);
+      else
+        hipLaunchKernelGGL((k_fz<P, C, false, TS, true>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0,
+                           nb, partial, iscale, twh, twf);
+      return hipGetLastError();
+    }
+  }
   if (rbub)
     hipLaunchKernelGGL((k_fz<P, C, true, TS>), dim3(nblk), dim3(256), 0, s, Tbuf, zre, rbub, nrows, rc, r0, nb,
                        partial, iscale, twh, twf);
@@ -1511,11 +1527,11 @@ hipError_t launch_fx(int N, const float2* S, const float2* D0, const float2* DN,
   }
 }
 
-int fz_blocks(int N, bool half, bool rb) {
+int fz_blocks(int N, bool half, bool rb, bool fbar) {
   switch (N) {
 #define X(NN) \
   case NN:    \
-    return fz_grid<NN>(half, rb);
+    return fz_grid<NN>(half, rb, fbar);
     PRC_FOR_EACH_N(X)
 #undef X
     default:
@@ -1524,16 +1540,16 @@ int fz_blocks(int N, bool half, bool rb) {
 }
 
 hipError_t launch_fz(int N, const void* Tbuf, bool half, float* zre, float* rbub, const RadiusConst* rc, int r0,
-                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf,
+                     int nb, double* partial, int nblk, float iscale, const float2* twh, const float2* twf, bool fbar,
                      hipStream_t s) {
   if (nb < 1 || nb > kFusedMaxRadii) return hipErrorInvalidValue;
   switch (N) {
 #define X(NN)                                                                                                 \
   case NN:                                                                                                    \
     return half ? fz_impl<NN, __half2>(static_cast<const __half2*>(Tbuf), zre, rbub, rc, r0, nb, partial, nblk, \
-                                       iscale, twh, twf, s)                                                   \
+                                       iscale, twh, twf, false, s)                                            \
                 : fz_impl<NN, float2>(static_cast<const float2*>(Tbuf), zre, rbub, rc, r0, nb, partial, nblk,   \
-                                      iscale, twh, twf, s);
+                                      iscale, twh, twf, fbar, s);
     PRC_FOR_EACH_N(X)
 #undef X
     default:

@@ -89,6 +89,9 @@ bool use_split(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_MIXED &
 // otherwise the pass runs its vector plan
 bool has_matrix_x(const prc_d2z_params& p) { return use_split(p) || (half_storage(p) && fx_mfma_supported(p.N, true)); }
 bool has_matrix_z(const prc_d2z_params& p) { return use_split(p) || (half_storage(p) && fz_mfma_supported(p.N, true)); }
+// the mixed mode's vector z pass evaluates the barrier as one FP32 FMA, as its
+// matrix-core z pass does; the fused FP32 mode keeps the reference's double one
+bool fp32_barrier(const prc_d2z_params& p) { return p.mode == PRC_D2Z_FUSED_MIXED; }
 
 }  // namespace
 
@@ -218,7 +221,7 @@ int forward(prc_d2z_ctx* c, float dinit) {
 int default_fz_blocks(const prc_d2z_params& p, int zplan) {
   const bool half = half_storage(p), rb = p.RecordBubbleSizes != 0;
   if (zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_z(p)) return use_split(p) ? fz_split_blocks(rb) : fz_mfma_blocks(rb);
-  return fz_blocks(p.N, half, rb);
+  return fz_blocks(p.N, half, rb, fp32_barrier(p));
 }
 bool matrix_x(const prc_d2z_ctx* c) { return c->xplan == PRC_D2Z_PLAN_MATRIX && has_matrix_x(c->p); }
 bool matrix_z(const prc_d2z_ctx* c) { return c->zplan == PRC_D2Z_PLAN_MATRIX && has_matrix_z(c->p); }
@@ -330,7 +333,7 @@ int fused_yz(prc_d2z_ctx* c, int r0, int nb, float2* T, hipStream_t s) {
                 return launch_fz_mfma(N, T, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
                                       c->d_tw_full, c->d_mats, s);
               return launch_fz(N, T, half, c->d_zre, rb, c->d_rc, r0, nb, c->d_part, c->nblk_fz, ms.inv,
-                               c->d_st_half, c->d_tw_full, s);
+                               c->d_st_half, c->d_tw_full, fp32_barrier(c->p), s);
             }),
             "fused z pass");
   return PRC_D2Z_OK;
