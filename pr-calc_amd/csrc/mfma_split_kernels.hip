@@ -96,8 +96,9 @@ __device__ __forceinline__ SplitOp load_mat(const half8_t* __restrict__ mats, in
 // 2^11 (x - hi) is formed by one mixed-precision FMA from the FP16 high part,
 // fma(hi, -2^11, 2^11 x), and rounded to FP16 in the same instruction
 // (v_fma_mixlo/mixhi_f16): 2^11 (x - hi) is exact in FP32, so the FMA's single
-// rounding gives the same value as the subtract and scale (the x pass 2.5-3.5%
-// faster at 1024^3 than with the three-instruction form).
+// rounding gives the same value as the subtract and scale, with 52 fewer vector
+// instructions per radius in the x pass (the pass's time is set by its stores
+// and did not change measurably).
 __device__ __forceinline__ SplitOp split_b(const float2 (&x)[4]) {
   SplitOp o;
 #pragma unroll
@@ -138,11 +139,11 @@ __device__ __forceinline__ float4_t mfma_split(const SplitOp& a, const SplitOp& 
 // 2^15.  The outputs are multiplied by 1/bs (exact) before they are stored.
 // The two correction products keep their own accumulator and join the main
 // product in one FP32 FMA.  Accumulating all three products in one MFMA
-// accumulator measured 1-1.5% faster but raised the largest error of a 1024^3
-// smoothed field from 2.7e-6 to 5.3e-5 - 5.8e-5 of its rms, whether the low
-// parts were left unscaled or the matrices (x 2^5) and the data (<= 2^5) were
-// scaled so that every part is a normal FP16 number: the matrix cores' sum
-// of the large products and the small accumulator input is not FP32-exact.
+// accumulator raised the largest error of a 1024^3 smoothed field from 2.7e-6
+// to 5.3e-5 - 5.8e-5 of its rms, whether the low parts were left unscaled or
+// the matrices (x 2^5) and the data (<= 2^5) were scaled so that every part is
+// a normal FP16 number: the matrix cores' sum of the large products and the
+// small accumulator input is not FP32-exact.
 //
 // LDS exchange rows hold 8 columns of float2 (64 B); row `row` is stored at
 // row ^ bit4(row), which puts the two halves of every quarter wave (butterfly
