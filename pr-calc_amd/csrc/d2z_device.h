@@ -111,6 +111,34 @@ __device__ __forceinline__ float filter_from_table(float km, float dkR, float dk
   return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x) * invN3;
 }
 
+// One radius's W table from a |k| table: wt[q] = filter_from_table(kt[q], ...)
+// for q < n, ITERS strided entries per thread of NT (ITERS NT >= n).  With an
+// interpolation table (M > 0) a thread's entries are independent and written
+// without branches, so their LDS reads overlap instead of queueing one after
+// the other; the values are filter_from_table's bit for bit.
+template <int NT, int ITERS>
+__device__ __forceinline__ void fill_wtable(float* wt, const float* kt, int n, float dkR, float dkRh, float scale,
+                                            const float4* tab, int M, int tid) {
+  if (ITERS <= 8 && M > 0) {
+#pragma unroll
+    for (int k = 0; k < ITERS; ++k) {
+      const int q = tid + k * NT;
+      if (q < n) {
+        const float km = kt[q];
+        const float u = km * dkRh;
+        int i = (int)u;
+        i = i < M - 1 ? i : M - 1;
+        const float t = u - (float)i;
+        const float4 c = tab[i];
+        const float w = fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x) * scale;
+        wt[q] = km == 0.0f ? 0.0f : w;
+      }
+    }
+  } else {
+    for (int q = tid; q < n; q += NT) wt[q] = filter_from_table(kt[q], dkR, dkRh, scale, tab, M);
+  }
+}
+
 // The same in two steps for tables that reuse |k| over radii:
 // kmag_grid(m) = |k| / dk, filter_of_kmag(kmag_grid(m), dkR, invN3) ==
 // filter_coef(m, dkR, invN3) bit for bit.

@@ -18,6 +18,20 @@
 
 namespace prc {
 
+// LDS loads whose values are first used after a block barrier must still be
+// done before it when other waves write the same LDS after that barrier.  The
+// backend may sink such loads below __syncthreads (seen in the persistent y
+// pass: the last gather of one tile moved past the barrier that frees the
+// buffer, so that other waves' writes of the next tile raced with it, and a
+// few waves of one tile came out wrong in about 5% of runs at 128^3).
+// lds_hold(v) before the barrier takes the values as inputs of an empty
+// volatile asm, which keeps their loads (and what they feed) above it.
+template <int E>
+__device__ __forceinline__ void lds_hold(const float2 (&v)[E]) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) asm volatile("" ::"v"(v[e].x), "v"(v[e].y));
+}
+
 // floor(log2(n)) for n >= 1
 constexpr int ilog2(int n) { return n > 1 ? 1 + ilog2(n / 2) : 0; }
 
@@ -383,6 +397,7 @@ __device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float
     stockham_stage<P, STAGE, DIR, false, V, TWT>(v, i, tw, lds);
     if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     lds_gather<P>(v, i, lds);
+    if constexpr (!V::kWaveLocal) lds_hold(v);  // the gather is done before the barrier
     if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     pencil_fft<P, DIR, STAGE + 1, V, TWT>(v, i, tw, lds);
   }

@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(kThreads)
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
     const float dkRh = dkR * winvh;
-    for (int q = tid; q < ntab; q += kThreads) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, invN3, wsh, wM);
+    fill_wtable<kThreads, ((kCF + 1) * kNF + kThreads - 1) / kThreads>(wtab, ktab, ntab, dkR, dkRh, invN3, wsh, wM,
+                                                                        tid);
     __syncthreads();
     // stage A: filter, radix 16 on the matrix cores, outputs to bufX
 #pragma unroll
@@ -560,6 +561,7 @@ __global__ void __launch_bounds__(512)
         v[m] = cadd(u0, u1);
         v[m + 4] = csub(u0, u1);
       }
+      lds_hold(v);      // the reads of bufX stay above the barrier
       __syncthreads();  // the raw buffer is reloaded next radius
       const BarrierF32 bk = s_bk[q];
       const float R = rc[r0 + q].R;

@@ -242,7 +242,8 @@ __global__ void __launch_bounds__(kFxThreads)
     asm volatile("" : "+v"(zo));
     const float dkR = dk * rc[r0 + rl].R;
     const float dkRh = dkR * winvh;
-    for (int q = tid; q < ntab; q += kFxThreads) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, wscale, wsh, wM);
+    fill_wtable<kFxThreads, ((kCF + 1) * kNF + kFxThreads - 1) / kFxThreads>(wtab, ktab, ntab, dkR, dkRh, wscale, wsh,
+                                                                              wM, tid);
     __syncthreads();  // (1) W table complete; the previous radius's stage-B reads of bufX are done
     // stage A: filter, split, radix 16 on the matrix cores, outputs to bufX
 #pragma unroll
@@ -485,6 +486,7 @@ __global__ void __launch_bounds__(kFzThreads) __attribute__((amdgpu_waves_per_eu
         v[mm] = cadd(u0, u1);
         v[mm + 4] = csub(u0, u1);
       }
+      lds_hold(v);      // the reads of bufX stay above the barrier
       __syncthreads();  // the raw buffer is reloaded next radius
       const BarrierF32 bk = s_bk[q];
       const float R = rc[r0 + q].R;

@@ -181,9 +181,13 @@ __global__ void __launch_bounds__(P::T* C)
     k_tile_ip(TS* data, long long sA, long long sO, long long sCB, int nOuter, long long ntiles, float oscale,
               const float2* __restrict__ tw) {
   constexpr int L = P::L, T = P::T, E = P::E;
+  constexpr int NTW = plan_tw_size<P>();
   __shared__ float2 smem[L * C];
+  __shared__ float2 s_tw[NTW > 0 ? NTW : 1];  // the plan's twiddles, copied once per (persistent) block
   const int tid = threadIdx.x;
   const int c = tid % C, i = tid / C;
+  for (int q = tid; q < NTW; q += T * C) s_tw[q] = tw[q];
+  __syncthreads();
   const long long istep = (long long)T * sA;
   const long long lane_off = c + (long long)i * sA;
   auto tile_base = [&](long long vb) {
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(P::T* C)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld(raw + e, 0);
     if (vb + gridDim.x < ntiles) fetch(vb + gridDim.x);
-    pencil_fft<P, DIR>(v, i, tw, LdsPencil{smem + c, C});
+    pencil_fft<P, DIR>(v, i, s_tw, LdsPencil{smem + c, C});
     TS* op = data + tile_base(vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) st(op, e * istep, cscale(v[e], oscale));
@@ -461,7 +465,7 @@ __device__ __forceinline__ void fx_body(const float2* __restrict__ S, const floa
     const float dkRh = dkR * winvh;
     // tabulate W(|k| R)/N^3 for this block's columns (the exchange buffer is
     // free: pencil_fft's last LDS access precedes its last barrier)
-    for (int q = tid; q < ntab; q += NT) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, invN3, wsh, wM);
+    fill_wtable<NT, ((CF + 1) * NF + NT - 1) / NT>(wtab, ktab, ntab, dkR, dkRh, invN3, wsh, wM, tid);
     __syncthreads();
     float2 v[E];
 #pragma unroll
@@ -477,6 +481,7 @@ __device__ __forceinline__ void fx_body(const float2* __restrict__ S, const floa
         v[e] = make_float2(w0 * m[e].x - wn * b.y, w0 * m[e].y + wn * b.x);
       }
     }
+    lds_hold(v);      // the table reads stay above the barrier
     __syncthreads();  // table reads done before the transform reuses the buffer
     pencil_fft<P, +1>(v, i, tw, lds);
     TS* dst = tp + (long long)rl * plane;
