@@ -387,8 +387,11 @@ __device__ __forceinline__ void lds_gather(float2 (&v)[P::E], int i, V lds) {
 // Full pencil transform: registers in natural layout -> registers in natural
 // layout.  All threads of the block (of the wave, for a wave-local view) must
 // call it (it contains barriers); the LDS region may be reused right after it
-// returns.
-template <class P, int DIR, int STAGE = 0, class V, bool TWT = false>
+// returns.  TAIL = false leaves out the barrier after the last gather: the
+// caller then synchronises before anything rewrites the region (a loop over
+// tiles or radii syncs at the top of its body, which every wave reaches only
+// after its last stage has used the gathered values).
+template <class P, int DIR, int STAGE = 0, class V, bool TWT = false, bool TAIL = true>
 __device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float2* __restrict__ tw, V lds) {
   if constexpr (STAGE == P::S - 1) {
     stockham_stage<P, STAGE, DIR, true, V, TWT>(v, i, tw, lds);
@@ -397,9 +400,11 @@ __device__ __forceinline__ void pencil_fft(float2 (&v)[P::E], int i, const float
     stockham_stage<P, STAGE, DIR, false, V, TWT>(v, i, tw, lds);
     if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
     lds_gather<P>(v, i, lds);
-    if constexpr (!V::kWaveLocal) lds_hold(v);  // the gather is done before the barrier
-    if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
-    pencil_fft<P, DIR, STAGE + 1, V, TWT>(v, i, tw, lds);
+    if constexpr (TAIL || STAGE + 2 < P::S) {
+      if constexpr (!V::kWaveLocal) lds_hold(v);  // the gather is done before the barrier
+      if constexpr (V::kWaveLocal) wave_lds_sync(); else __syncthreads();
+    }
+    pencil_fft<P, DIR, STAGE + 1, V, TWT, TAIL>(v, i, tw, lds);
   }
 }
 

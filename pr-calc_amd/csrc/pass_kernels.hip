@@ -203,11 +203,12 @@ __global__ void __launch_bounds__(P::T* C)
   long long vb = blockIdx.x;
   if (vb < ntiles) fetch(vb);
   for (; vb < ntiles; vb += gridDim.x) {
+    __syncthreads();  // the previous tile's last gather is done (pencil_fft without its tail barrier)
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld(raw + e, 0);
     if (vb + gridDim.x < ntiles) fetch(vb + gridDim.x);
-    pencil_fft<P, DIR>(v, i, s_tw, LdsPencil{smem + c, C});
+    pencil_fft<P, DIR, 0, LdsPencil, false, false>(v, i, s_tw, LdsPencil{smem + c, C});
     TS* op = data + tile_base(vb);
 #pragma unroll
     for (int e = 0; e < E; ++e) st(op, e * istep, cscale(v[e], oscale));
@@ -461,10 +462,11 @@ __device__ __forceinline__ void fx_body(const float2* __restrict__ S, const floa
   __syncthreads();  // ktab and the W(x) table complete
 #pragma unroll 1
   for (int rl = 0; rl < nb; ++rl) {
+    __syncthreads();  // the last radius's final gather is done (pencil_fft without its tail barrier)
     const float dkR = dk * rc[r0 + rl].R;
     const float dkRh = dkR * winvh;
     // tabulate W(|k| R)/N^3 for this block's columns (the exchange buffer is
-    // free: pencil_fft's last LDS access precedes its last barrier)
+    // free after the barrier above)
     fill_wtable<NT, ((CF + 1) * NF + NT - 1) / NT>(wtab, ktab, ntab, dkR, dkRh, invN3, wsh, wM, tid);
     __syncthreads();
     float2 v[E];
@@ -483,7 +485,7 @@ __device__ __forceinline__ void fx_body(const float2* __restrict__ S, const floa
     }
     lds_hold(v);      // the table reads stay above the barrier
     __syncthreads();  // table reads done before the transform reuses the buffer
-    pencil_fft<P, +1>(v, i, tw, lds);
+    pencil_fft<P, +1, 0, LdsColPad<SH>, false, false>(v, i, tw, lds);
     TS* dst = tp + (long long)rl * plane;
 #pragma unroll
     for (int e = 0; e < E; ++e) st(dst, (long long)(i + e * T) * N * C, v[e]);
