@@ -174,8 +174,9 @@ __global__ void __launch_bounds__(kThreads)
   for (int rl = 0; rl < nb; ++rl) {
     const float dkR = dk * rc[r0 + rl].R;
     const float dkRh = dkR * winvh;
-    fill_wtable<kThreads, ((kCF + 1) * kNF + kThreads - 1) / kThreads>(wtab, ktab, ntab, dkR, dkRh, invN3, wsh, wM,
-                                                                        tid);
+    // (the unrolled fill_wtable measured slower here: it pushes this kernel
+    // from 8 to 20 bytes of spills)
+    for (int q = tid; q < ntab; q += kThreads) wtab[q] = filter_from_table(ktab[q], dkR, dkRh, invN3, wsh, wM);
     __syncthreads();
     // stage A: filter, radix 16 on the matrix cores, outputs to bufX
 #pragma unroll
