@@ -612,10 +612,27 @@ __device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __r
     bR0[e] = 0.f;
     bR1[e] = 0.f;
   }
+  // FP32 storage at N = 1024 (8-element pencils), no bubble radii: the next
+  // radius's row is loaded while this one is transformed (16 more VGPRs, 112
+  // -> 124, still 4 waves per SIMD; z pass 110 -> 107.5 ms per 1024^3 x 96
+  // step, same bits).  The other variants would lose a wave per SIMD.
+  constexpr bool kPre = !kH && !RB && E <= 8 && L == 512;
+  float2 nx[kPre ? E : 1];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) nx[e] = ld(Tbuf, zrow_offset<P, C>(i, e, r, nrows));
+  }
 #pragma unroll 1
   for (int q = 0; q < nb; ++q) {
     const TS* tp = Tbuf + (long long)q * plane;
     float2 v[E];
+    if constexpr (kPre) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = nx[e];
+      const TS* tn = Tbuf + (long long)min(q + 1, nb - 1) * plane;  // the last radius reloads its own row
+#pragma unroll
+      for (int e = 0; e < E; ++e) nx[e] = ld(tn, zrow_offset<P, C>(i, e, r, nrows));
+    }
     // read per radius from LDS rather than held in registers across radii
     int zero = 0;
     asm volatile("" : "+s"(zero));
@@ -632,8 +649,10 @@ __device__ __forceinline__ void fz_group(const TS* __restrict__ Tbuf, float* __r
       }
     } else {
       float2 m[E];
+      if constexpr (!kPre) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+        for (int e = 0; e < E; ++e) v[e] = ld(tp, zrow_offset<P, C>(i, e, r, nrows));
+      }
       mirror_lanes<P>(v, m, i);
       c2r_pre<P>(v, m, i, s_twf + zero);
     }
