@@ -46,6 +46,27 @@ __device__ __forceinline__ long long xcd_tile(long long b, long long nblk) {
   return (b % 8) * (nblk / 8) + b / 8;
 }
 
+// Tile of block b for the x passes, whose tiles are numbered so that each
+// run of `group` consecutive tiles writes whole 128 B lines of the output
+// together (the column parts of a row, and the adjacent rows sharing a line
+// when rows are shorter than 128 B): a group runs on one XCD (its L2 merges
+// the partial lines) and consecutive groups on consecutive XCDs (blocks go to
+// the XCDs round-robin), so the resident blocks of all XCDs write one run of
+// consecutive lines of each output row.  For FP32 outputs this measured
+// faster than xcd_tile (each XCD on its own range of tiles), same bits: at
+// 1024^3 x 96 the split x pass 122-136 against 137-145 ms per step, the FP32
+// vector x pass 146-162 against 167-169, at 2048^3 x 128 2074 against 2136.
+// The FP16-output x passes measured slower with it (at 1024^3 98-106 against
+// 97-98 ms, at 2048^3 1871 against 1769) and keep xcd_tile.
+__device__ __forceinline__ long long line_tile(long long b, long long nblk, int group) {
+  if (nblk % (8LL * group) != 0) return b;
+  const long long slot = b >> 3;
+  return ((slot / group) * 8 + (b & 7)) * group + slot % group;
+}
+
+// Tiles per 128 B output line: nsub column parts of rows of row_bytes.
+constexpr int line_group(int nsub, int row_bytes) { return row_bytes >= 128 ? nsub : nsub * (128 / row_bytes); }
+
 // ---------------------------------------------------- top-hat filter W(kR)
 // sin and cos for 0 <= x < 2^12: Cody-Waite reduction by pi/2 in three parts
 // with FMA, then the single-precision minimax polynomials of the Cephes

@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(kFxThreads)
   const int g = l >> 4, n = l & 15;
   const int c = n & 7, sel = n >> 3;  // MFMA column n = (column c, butterfly parity sel)
 
-  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  const long long tile = line_tile(blockIdx.x, gridDim.x, line_group(kNSUB, kC * (int)sizeof(float2)));
   const int sub = (int)(tile % kNSUB);
   const int ky = (int)((tile / kNSUB) % kN);
   const int kzb = (int)(tile / (kNSUB * kN));

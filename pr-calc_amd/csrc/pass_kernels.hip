@@ -501,8 +501,11 @@ __global__ void __launch_bounds__(P::T* CF)
   __shared__ float ktab[FxLds<P, CF>::KTAB];  // |k|/dk of the table entries, same order
   extern __shared__ float4 wsh[];              // W(x) interpolation table (wM intervals)
   for (int q = threadIdx.x; q < wM; q += P::T * CF) wsh[q] = wcoef[q];
-  fx_body<P, C, CF, TS>(S, D0, DN, Tbuf, dk, invN3, rc, r0, nb, tw, wsh, wM, winvh, xcd_tile(blockIdx.x, gridDim.x),
-                        smem, ktab);
+  // FP32 outputs: consecutive lines over the XCDs; FP16: one range per XCD
+  const long long tile = std::is_same<TS, float2>::value
+                             ? line_tile(blockIdx.x, gridDim.x, line_group(C / CF, C * (int)sizeof(TS)))
+                             : xcd_tile(blockIdx.x, gridDim.x);
+  fx_body<P, C, CF, TS>(S, D0, DN, Tbuf, dk, invN3, rc, r0, nb, tw, wsh, wM, winvh, tile, smem, ktab);
 }
 
 // columns per fused-x block: keep the block at <= 512 threads
