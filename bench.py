@@ -81,17 +81,29 @@ def kernel_alg_bytes(name: str, N: int, nb: int, store_bytes: int) -> float:
     return table.get(name, 0.0) * cells
 
 
-def traffic_from_profiles(kernel: str, workload: str, mode: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE, calibrated), if any."""
+def traffic_profile(workload: str, mode: str) -> dict:
+    """The committed rocprofv3 counter summary of a workload and mode
+    (FETCH_SIZE and WRITE_SIZE per launch, calibrated), or {}."""
     f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
     if not f.exists():
-        return None
+        return {}
     try:
-        d = json.loads(f.read_text())
-        return d.get(kernel, {}).get("bytes_per_launch")
+        return json.loads(f.read_text())
     except Exception:
+        return {}
+
+
+def measured_bytes(entry, alg_per_launch: float):
+    """HBM bytes of one launch at this run's launch size: the profiled
+    launch's measured / algorithmic ratio times this launch's algorithmic
+    bytes (the profile may hold another number of radii per launch); the
+    profiled bytes for a kernel without a byte model; None without an entry."""
+    if not entry:
         return None
+    r = entry.get("ratio")
+    if r is not None and alg_per_launch:
+        return r * alg_per_launch
+    return entry.get("bytes_per_launch")
 
 
 def cpu_baseline_sample(delta: np.ndarray, lp, nthreads: int) -> dict:
@@ -231,21 +243,28 @@ def main():
     fx_nb = plan["x_radii_per_launch"] if plan else FX_MAX_RADII
     store = STORE_BYTES[args.mode]
 
-    def kernel_roof(name):
+    def alg_step_of(name):
         """Algorithmic bytes of a kernel over the step (the last fused batch may
         carry fewer radii; the x pass fills a batch in launches of at most
-        FX_MAX_RADII, each reading the spectrum) over its event time."""
-        k_ms, k_n = kernels[name]
+        x_radii_per_launch, each reading the spectrum)."""
+        k_n = kernels[name][1]
         if args.mode == "plain" or name not in ("fx_filter", "inv_y", "fz_barrier"):
-            alg_step = kernel_alg_bytes(name, N, 1, store) * k_n
-        else:
-            launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
-            if name == "fx_filter":
-                launches = [min(fx_nb, n - o) for n in launches for o in range(0, n, fx_nb)]
-            alg_step = sum(kernel_alg_bytes(name, N, n, store) for n in launches)
+            return kernel_alg_bytes(name, N, 1, store) * k_n
+        launches = [min(nb, NS - r0) for r0 in range(0, NS, nb)]
+        if name == "fx_filter":
+            launches = [min(fx_nb, n - o) for n in launches for o in range(0, n, fx_nb)]
+        return sum(kernel_alg_bytes(name, N, n, store) for n in launches)
+
+    tprof = traffic_profile(args.workload, args.mode)
+
+    def kernel_roof(name):
+        """A kernel's algorithmic bytes over its event time, and its measured
+        HBM bytes per launch."""
+        k_ms, k_n = kernels[name]
+        alg_step = alg_step_of(name)
         achieved = alg_step / (k_ms * 1e-3) / 1e9 if alg_step else None
         return {"achieved": achieved, "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                "traffic": traffic_from_profiles(name, args.workload, args.mode),
+                "traffic": measured_bytes(tprof.get(name), alg_step / k_n),
                 "avg_launch_ms": k_ms / k_n, "alg_bytes_per_launch": alg_step / k_n, "ms_per_step": k_ms}
 
     dr = kernel_roof(dom_name)
@@ -261,7 +280,13 @@ def main():
         roof["fused_kernels"] = {k: kernel_roof(k) for k in ("fx_filter", "inv_y", "fz_barrier") if k in kernels}
     # measured HBM bytes of the whole step (rocprof FETCH_SIZE + WRITE_SIZE per
     # kernel launch, committed under profiles/) over the step time
-    step_bytes = traffic_per_step(args.workload, args.mode, kernels)
+    step_bytes = 0.0
+    for name, (_, k_n) in kernels.items():
+        b = measured_bytes(tprof.get(name), alg_step_of(name) / k_n)
+        if b is None:
+            step_bytes = None
+            break
+        step_bytes += b * k_n
     if step_bytes:
         roof["path_traffic_bytes"] = step_bytes
         roof["path_traffic_frac"] = step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
@@ -348,26 +373,6 @@ def path_rate(lp, mode: str, steps: int, tune: bool = True) -> dict:
     return {"value": N ** 3 * NS * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
             "dtype": DTYPES[mode], "fft_plans": {"x": "matrix" if plan["x_plan"] else "vector",
                                                   "z": "matrix" if plan["z_plan"] else "vector"}}
-
-
-def traffic_per_step(workload: str, mode: str, kernels: dict):
-    """Sum over the step's kernels of the measured bytes per launch times the
-    launches per step, from the committed traffic profile (None if a kernel
-    of the step has no entry)."""
-    f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
-    if not f.exists():
-        return None
-    try:
-        d = json.loads(f.read_text())
-    except Exception:
-        return None
-    total = 0.0
-    for name, (_, n) in kernels.items():
-        b = d.get(name, {}).get("bytes_per_launch")
-        if b is None:
-            return None
-        total += b * n
-    return total
 
 
 def plain_fp32_rate(lp, steps: int) -> dict:
