@@ -1454,9 +1454,16 @@ hipError_t launch_inv_z_h(int N, const void* in, float* out, float iscale, const
 
 hipError_t launch_tile(int N, int dir, const float2* in, float2* out, const TileMap& mp, const float2* tw,
                        hipStream_t s) {
-  // FP32 tiles keep one tile per block: the persistent kernel of the FP16
-  // path measured slower for FP32 pencils of 8 values (N = 512) and spills for
-  // 16 (N = 1024, 2048)
+  // The in-place inverse y pass at N = 1024 and 2048 runs the persistent
+  // kernel (the next tile's 16 FP32 pairs per thread loaded into registers
+  // while the current one is transformed: 112-113 VGPRs, no spills): same
+  // bits, 141.1-141.5 -> 134.4-138.2 ms per 1024^3 x 96 step and 1548-1555
+  // -> 1457-1459 ms per 2048^3 x 128 step.  Other sizes keep one tile per
+  // block (the persistent kernel measured slower for FP32 pencils of 8 values
+  // at N = 512).
+  if ((N == 1024 || N == 2048) && dir > 0 && in == out && mp.in_sA == mp.out_sA && mp.in_sO == mp.out_sO)
+    return N == 1024 ? tile_ip_impl<1024, +1, float2>(out, mp, 1.0f, tw, s)
+                     : tile_ip_impl<2048, +1, float2>(out, mp, 1.0f, tw, s);
   switch (N) {
 #define X(NN)                                                                          \
   case NN:                                                                             \
