@@ -553,3 +553,33 @@ def test_autotune_plans_1024_mixed():
         print(k, "max|dz| vs default plan", dz)
         assert dz <= 2e-3
     assert float(np.max(np.abs(z_tuned.astype(np.float64) - z_default))) <= 2e-3
+
+
+def test_persistent_y_pass_repeated_runs_1024():
+    """At N = 1024 the FP32 y pass runs the persistent tile kernel: each block
+    walks tiles, loads the next one into registers while it transforms the
+    current one, and frees the LDS for the next tile with the barrier at the
+    top of its loop.  A race there would change a few cells from run to run
+    (as an earlier form of the FP16 kernel did at 128^3): three fresh 1024^3
+    contexts of the fused FP32 and mixed paths, 8 radii, each run twice, give
+    the same zreion and smoothed field (radius 7) in every run."""
+    from prc_d2z import D2ZParams
+    N = 1024
+    for mode in (FUSED_FP32, FUSED_MIXED):
+        ref = None
+        for t in range(3):
+            announce(f"1024^3 repeated runs mode={mode} context {t}")
+            lp = D2ZParams(N=N, BoxSize=2000.0, Nscales=8, Rmax=2000.0 / N * (120.0 / (2000.0 / N)) ** (7 / 95),
+                           zeta=40.0, M_min=1.4706e8, mode=mode)
+            with D2Z(lp) as d:
+                d.generate_delta(1)
+                d.run()
+                z1 = d.zreion()
+                d.run()
+                z2 = d.zreion()
+                f7 = d.smoothed_field(7)
+            if ref is None:
+                ref = (z1, f7)
+            assert np.array_equal(z1, ref[0]) and np.array_equal(z2, ref[0]), (mode, t)
+            assert np.array_equal(f7, ref[1]), (mode, t)
+            del z1, z2, f7
