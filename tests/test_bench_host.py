@@ -60,7 +60,7 @@ def test_measured_bytes_follow_the_launch_size():
     assert bench.measured_bytes({}, alg8) is None
 
 
-def test_traffic_profile_of_a_missing_workload_is_empty(tmp_path):
+def test_traffic_profile_of_a_missing_workload_is_empty():
     assert bench.traffic_profile("no_such_workload", "mixed") == {}
     # the committed profiles are plain JSON with the commands that made them
     for mode in ("mixed", "fused", "half"):
